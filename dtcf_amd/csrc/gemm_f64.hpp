@@ -1,0 +1,44 @@
+// Internal launch interface of the FP64 MFMA GEMM (gemm_f64.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace dtcf {
+
+// Operand addressing modes (rows of the view x k):
+//   kDK : dense, (r, c) at r*ld + c   -- k contiguous
+//   kDMN: dense, (r, c) at c*ld + r   -- rows contiguous
+//   kVK : offset tables, pairs (c, c+1) contiguous
+//   kVMN: offset tables, pairs (r, r+1) contiguous
+enum { kDK = 0, kDMN = 1, kVK = 2, kVMN = 3 };
+
+// Tile configurations (BM x BN), selectable per launch by the autotuner.
+enum { kCfg128x128 = 0, kCfg192x96 = 1, kCfg128x144 = 2, kCfg64x64 = 3, kCfgCount = 4 };
+
+struct OpArgs {
+    int64_t ld;
+    const int64_t *rofs;
+    const int64_t *cofs;
+    const int64_t *blk;
+    FastDiv div_rpb, div_cpb;
+    int64_t ncb;
+    int32_t vec2;  // 1: (r,c),(r,c+1) [k-mode] or (r,c),(r+1,c) [mn-mode] are adjacent
+                   //    and 16-byte aligned wherever both are in range
+};
+
+struct GemmArgs {
+    int64_t m, n, k;
+    double alpha;
+    const double *A;
+    const double *B;
+    double *C;
+    OpArgs a, b, c;
+    int32_t amode, bmode, cview;
+    uint32_t tiles_m, tiles_n;
+};
+
+int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
+
+}  // namespace dtcf

@@ -1,0 +1,208 @@
+/*
+ * dtcf_local.h -- C ABI of the MI355X (gfx950) local-contraction engine.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * DTCF_OK (0) or a negative dtcf_status; dtcf_last_error() gives the text of
+ * the last failure on the calling thread.  Device-pointer entry points take a
+ * hipStream_t passed as `void *` (NULL = the default stream) and only enqueue
+ * work; host-pointer entry points ("_host") stage through device buffers and
+ * return after the result is back in host memory.
+ *
+ * Each entry point names the reference interface it replaces (file:line of
+ * samyam/DTCF).  The C++ drop-in symbols with the reference's exact
+ * signatures are declared in dtcf_dropin.h and forward to these.
+ */
+#ifndef DTCF_LOCAL_H
+#define DTCF_LOCAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DTCF_OK = 0,
+    DTCF_ERR_INVALID = -1,     /* bad shape, permutation, pointer or descriptor */
+    DTCF_ERR_UNSUPPORTED = -2, /* e.g. DLTC transpose of a rank other than 2 or 4 */
+    DTCF_ERR_HIP = -3,         /* a HIP runtime call failed */
+    DTCF_ERR_NOMEM = -4,       /* device allocation failed */
+    DTCF_ERR_BUDGET = -5,      /* serving: request larger than the HBM budget */
+    DTCF_ERR_NOTREADY = -6     /* serving: result not complete yet */
+} dtcf_status;
+
+#define DTCF_MAX_DIMS 8
+
+/* ------------------------------------------------------------------ */
+/* Library                                                             */
+/* ------------------------------------------------------------------ */
+const char *dtcf_last_error(void);
+/* Select the device for the calling thread and warm up the kernels. */
+int dtcf_init(int device);
+/* Version of this ABI (major*100 + minor). */
+int dtcf_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* 1. Strided permute-copy (device pointers)                           */
+/*                                                                     */
+/* Replaces the per-tile permutation of RRR transpose (helper.cc:275,  */
+/* called at contraction.cc:1843,1901,2705,2825), the DLTC kernels     */
+/* transpose_<perm> (DLTC/transp-kernel-seq.cc, dispatched by          */
+/* DLTC/transp.cc:3-59) and, with a row pitch in ostr, the big-matrix  */
+/* pack/unpack of create_big_matrix / revert_big_matrix                 */
+/* (transpose_helper.cc:149-159, 393-403).                               */
+/*                                                                     */
+/* For every multi-index x over `dims` dims of extent ext[]:           */
+/*   out[dst(t) + sum x[n]*ostr[n]] = alpha * in[src(t) + sum x[n]*istr[n]]  */
+/*                                    (+ beta * old out value if beta != 0) */
+/* for t in [0, ntiles); src(t) = src_off ? src_off[t] : t*src_step and  */
+/* dst(t) likewise.  src_off/dst_off are device arrays.  With alpha=1,  */
+/* beta=0 the copy is bit-exact.  Output elements must not alias each   */
+/* other or the input.                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t dims;                 /* 1..DTCF_MAX_DIMS */
+    int64_t ext[DTCF_MAX_DIMS];   /* extent of each iterated dim */
+    int64_t istr[DTCF_MAX_DIMS];  /* input stride (elements) of each dim */
+    int64_t ostr[DTCF_MAX_DIMS];  /* output stride (elements) of each dim */
+} dtcf_strided;
+
+int dtcf_permute_copy_f64(const double *in, double *out, const dtcf_strided *desc,
+                          int64_t ntiles, const int64_t *src_off, int64_t src_step,
+                          const int64_t *dst_off, int64_t dst_step, double alpha,
+                          double beta, void *stream);
+
+/* Convenience: permute `ntiles` consecutive dense row-major tiles of extents
+ * w[0..dims-1].  out_src[n] = input dim that feeds output dim n (the DLTC
+ * convention, s = position(y, x), DLTC/transp.h:8-17).  The RRR convention
+ * pmap (input dim i -> output position pmap[i], helper.cc:289-292) converts
+ * as out_src[pmap[i]] = i. */
+int dtcf_permute_tiles_f64(const double *in, double *out, int dims, const int64_t *w,
+                           const int32_t *out_src, int64_t ntiles, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 2. FP64 GEMM on MFMA (device pointers)                               */
+/*                                                                     */
+/* Replaces local_contraction (localcontract.cc:3-21) and DLTC compute  */
+/* (DLTC/compute.cc:3-36): row-major C[m x n] += alpha*op(A)*op(B)      */
+/* (beta is always 1, as in both reference call sites).                 */
+/* op(A) = A [m x k] (lda >= k) or A^T with A [k x m] (lda >= m);       */
+/* op(B) = B [k x n] (ldb >= n) or B^T with B [n x k] (ldb >= k).       */
+/* These are the intended leading dimensions; the reference passes     */
+/* lda = n_k and ldb = n_b whatever the flags (localcontract.cc:7-9).   */
+/* ------------------------------------------------------------------ */
+int dtcf_gemm_f64(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                  const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                  int64_t ldc, void *stream);
+
+/* A 2-D view of a tensor held as tiles: element (r, c) of a rows x cols
+ * view lives at element offset
+ *   rofs[r] + cofs[c] + (blk ? blk[(r / rpb) * ncb + c / cpb] : 0)
+ * of the tensor's buffer.  This is how the permutation and the
+ * create_big_matrix pack (transpose_helper.cc:20-171) are fused into the
+ * GEMM operand loads instead of being materialised.  A view object owns
+ * device copies of the tables; dtcf_view_create validates them (every
+ * offset >= 0) and records which pairs of elements are adjacent so the
+ * kernels can use 16-byte loads. */
+typedef struct dtcf_view_s *dtcf_view;
+
+int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int64_t *cofs,
+                     const int64_t *blk, int64_t rpb, int64_t cpb, dtcf_view *out);
+int dtcf_view_destroy(dtcf_view v);
+/* 1 + largest element offset the view can address (buffer size needed). */
+int64_t dtcf_view_span(dtcf_view v);
+
+/* C_view[m x n] += alpha * A_view[m x k] * B_view[n x k]^T: the RRR form in
+ * which both packed operands have the contraction index as columns
+ * (contraction.cc:1956, kevin_dgemm(..., at=0, bt=1, 1.0)).  A NULL view
+ * handle for C means C is dense row-major with leading dimension ldc. */
+int dtcf_gemm_view_f64(int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                       dtcf_view av, const double *B, dtcf_view bv, double *C, dtcf_view cv,
+                       int64_t ldc, void *stream);
+
+/* Gather a view into a dense row-major matrix (create_big_matrix pack,
+ * transpose_helper.cc:149-159): dst[r*ld + c] = src[view(r, c)]. */
+int dtcf_view_gather_f64(const double *src, dtcf_view v, double *dst, int64_t ld, void *stream);
+/* Scatter back (revert_big_matrix, transpose_helper.cc:393-403):
+ * dst[view(r, c)] = src[r*ld + c]. */
+int dtcf_view_scatter_f64(const double *src, int64_t ld, dtcf_view v, double *dst, void *stream);
+
+/* Tile-shape selection for the GEMM kernels: 0 = 128x128, 1 = 192x96,
+ * 2 = 128x144, 3 = 64x64, -1 = automatic (autotuned table). */
+int dtcf_gemm_set_config(int cfg);
+
+/* Host-pointer variant of dtcf_gemm_f64 for the drop-in local_contraction:
+ * copies A, B, C to the device, runs, copies C back. */
+int dtcf_gemm_f64_host(int transa, int transb, int64_t m, int64_t n, int64_t k,
+                       double alpha, const double *A, int64_t lda, const double *B,
+                       int64_t ldb, double *C, int64_t ldc);
+
+/* Host-pointer permute of consecutive dense tiles (drop-in transpose). */
+int dtcf_permute_tiles_f64_host(const double *in, double *out, int dims, const int64_t *w,
+                                const int32_t *out_src, int64_t ntiles);
+
+/* ------------------------------------------------------------------ */
+/* 3. One rank's local step of an RRR contraction C += alpha * A * B    */
+/*                                                                     */
+/* Replaces, for one SUMMA step: contr_list (contraction.cc:2553-2572), */
+/* generate_permutation_map (contraction.cc:164-282), the C pre/post   */
+/* transposes and pack (contraction.cc:2692-2830), transpose_and_dgemm  */
+/* (contraction.cc:1779-1976) with get_symm_permutation / compose_p_maps */
+/* (contraction.cc:121-160) and create/revert_big_matrix                */
+/* (transpose_helper.cc:20-409).                                        */
+/* ------------------------------------------------------------------ */
+/* A tensor held as RRR tiles (tensor.h:29-421): `ntiles` row-major tiles of
+ * extents block_range[], tile t at tiles + t*prod(block_range), with virtual
+ * tile address tile_addr[t*dims ...] (host array).  sym_group[d] is 0 for an
+ * 'a'/'A' dim, 1 for 'b'/'B', 3 for a non-symmetric dim (tensor.cc:93-164);
+ * tiles whose address is increasing inside a group are bounced copies whose
+ * data is permuted back without a sign (contraction.cc:132-160). */
+typedef struct {
+    int32_t dims;
+    int32_t vgrid[DTCF_MAX_DIMS];
+    int32_t block_range[DTCF_MAX_DIMS];
+    int32_t sym_group[DTCF_MAX_DIMS];
+    int64_t ntiles;
+    const int32_t *tile_addr; /* host */
+    double *tiles;            /* device */
+} dtcf_tileset;
+
+enum {
+    DTCF_STEP_LITERAL = 0, /* permute tiles, pack, GEMM, unpack, permute back */
+    DTCF_STEP_PACKED = 1,  /* one permute+pack gather per operand, GEMM, scatter */
+    DTCF_STEP_FUSED = 2    /* one GEMM reading the tiles through offset tables */
+};
+
+typedef struct dtcf_step_s *dtcf_step;
+
+/* Index strings are comma separated, as Contraction::contract takes them
+ * (e.g. "i,k,a,c").  The tilesets' device buffers must stay valid for the
+ * life of the plan; the plan owns its workspaces. */
+int dtcf_step_plan(const dtcf_tileset *A, const char *a_idx, const dtcf_tileset *B,
+                   const char *b_idx, const dtcf_tileset *C, const char *c_idx, int mode,
+                   dtcf_step *out);
+/* Enqueue the step on `stream`: C tiles += alpha * contraction. */
+int dtcf_step_run(dtcf_step s, double alpha, void *stream);
+/* n_a, n_b, n_k per tile (generate_permutation_map) and the big-matrix
+ * GEMM sizes M, N, K. */
+int dtcf_step_info(dtcf_step s, int64_t *mnk_tile, int64_t *mnk_big);
+int dtcf_step_destroy(dtcf_step s);
+
+/* ------------------------------------------------------------------ */
+/* 4. FP64 reductions and amplitude update (new; no reference          */
+/*    counterpart -- the reference stops at the contraction).          */
+/*    Layouts: R, T as [v, v, o, o]; V as [o, o, v, v].                 */
+/* ------------------------------------------------------------------ */
+/* T[a,b,i,j] = R[a,b,i,j] / (eo[i] + eo[j] - ev[a] - ev[b]) */
+int dtcf_denom_update_f64(const double *R, double *T, const double *eo, const double *ev,
+                          int64_t o, int64_t v, void *stream);
+/* *out = 1/4 sum_{ijab} V[i,j,a,b] * T[a,b,i,j]   (deterministic order) */
+int dtcf_energy_f64(const double *V, const double *T, int64_t o, int64_t v, double *out_dev,
+                    void *stream);
+/* *out = sqrt(sum R^2)   (deterministic order) */
+int dtcf_norm2_f64(const double *R, int64_t n, double *out_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DTCF_LOCAL_H */

@@ -1,0 +1,399 @@
+"""CPU oracle for DTCF's per-rank local contraction path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
+checker or as the timed CPU baseline.  The product package ``dtcf_amd`` never
+imports it.
+
+``liboracle.so`` (built from ``dtcf_oracle.c`` by ``oracle/Makefile``) holds
+the C restatements; this module wraps them with numpy and adds the few pieces
+of pure index bookkeeping that are easier to restate in Python:
+
+* ``permutation_maps``   -- contraction.cc:2553-2572 (contr_list) and
+  contraction.cc:164-282 (generate_permutation_map).
+* ``dltc_index_split``   -- DLTC/expr.cc:13-67 (Multiplication ctor).
+* ``rrr_tile_addresses`` -- tensor.cc:652-693 (generate_data order) with
+  tensor.cc:617-650 (isValid) and tensor.cc:761-783 (get_bound), one rank.
+
+Each function's pin (what checks it) is named in its docstring.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = {}
+
+_i64 = C.c_int64
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def build(ref: bool = False) -> None:
+    """Compile liboracle.so (and, when /root/reference exists, oracle/_ref)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    if ref and os.path.isdir("/root/reference/DLTC"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    L.orc_rrr_transpose.argtypes = [_dp, _dp, _ip, C.c_int, _ip]
+    L.orc_rrr_transpose_tiles.argtypes = [_dp, _dp, _ip, C.c_int, _ip, _i64, _i64, C.c_int]
+    L.orc_dltc_position.argtypes = [_ip, _ip, C.c_int, _ip]
+    L.orc_permute_by_source.argtypes = [_dp, _dp, _ip, C.c_int, _ip]
+    L.orc_dltc_transpose.argtypes = [_dp, _ip, _dp, _ip, C.c_int, _ip, C.c_int]
+    L.orc_dltc_transpose.restype = C.c_int
+    L.orc_block_rank.argtypes = [_ip, C.c_int, _ip]
+    L.orc_block_rank.restype = _i64
+    L.orc_block_addr.argtypes = [_i64, C.c_int, _ip, _ip]
+    L.orc_symm_permutation.argtypes = [_ip, _ip, C.c_int, _ip]
+    L.orc_compose_p_maps.argtypes = [_ip, _ip, C.c_int, _ip]
+    L.orc_big_matrix_order.argtypes = [C.c_int, _ip, _ip, _ip, _i64, C.c_int, _lp,
+                                       C.POINTER(_i64), C.POINTER(_i64)]
+    L.orc_big_matrix_order.restype = C.c_int
+    L.orc_big_matrix_pack.argtypes = [_dp, _dp, _lp, _i64, _i64, _i64, _i64, _i64, C.c_int]
+    L.orc_dgemm.argtypes = [C.c_int, C.c_int, _i64, _i64, _i64, C.c_double, _dp, _i64,
+                            _dp, _i64, _dp, _i64]
+    L.orc_dgemm_nt_blocked.argtypes = [_i64, _i64, _i64, C.c_double, _dp, _i64, _dp, _i64,
+                                       _dp, _i64, C.c_int]
+    L.orc_checkM.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int]
+    L.orc_checkS.argtypes = [_dp, _dp, C.c_int, C.c_int]
+    L.orc_drand48_fill.argtypes = [_dp, _i64, C.POINTER(C.c_uint64)]
+    L.orc_denom_update.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int]
+    L.orc_energy.argtypes = [_dp, _dp, C.c_int, C.c_int]
+    L.orc_energy.restype = C.c_double
+    L.orc_norm2.argtypes = [_dp, _i64]
+    L.orc_norm2.restype = C.c_double
+    L.orc_block_pairs.argtypes = [_i64, _lp, _lp, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp]
+    _LIB = L
+    return L
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+# --------------------------------------------------------------------------
+# permutations
+# --------------------------------------------------------------------------
+def rrr_transpose(tile, pmap, widths):
+    """helper.cc:275-522 -- input dim i -> output position pmap[i].
+    Pinned by brute force (tests/test_oracle.py)."""
+    tile = _f64(tile).ravel()
+    out = np.empty_like(tile)
+    lib().orc_rrr_transpose(tile, out, _i32(pmap), len(pmap), _i32(widths))
+    return out
+
+
+def rrr_transpose_tiles(tiles, pmap, widths, nthreads=0):
+    tiles = _f64(tiles).ravel()
+    bs = int(np.prod(widths))
+    out = np.empty_like(tiles)
+    lib().orc_rrr_transpose_tiles(tiles, out, _i32(pmap), len(pmap), _i32(widths),
+                                  tiles.size // bs, bs, nthreads)
+    return out
+
+
+def dltc_position(x, y):
+    """DLTC/transp.h:8-17."""
+    p = np.zeros(len(x), dtype=np.int32)
+    lib().orc_dltc_position(_i32(x), _i32(y), len(x), p)
+    return p
+
+
+def permute_by_source(buf, s, widths):
+    buf = _f64(buf).ravel()
+    out = np.empty_like(buf)
+    lib().orc_permute_by_source(buf, out, _i32(s), len(s), _i32(widths))
+    return out
+
+
+def dltc_transpose(buf_x, x, y, wx, q2_literal=False, out=None):
+    """DLTC/transp.cc:3-59.  Returns (status, buf_y).  Pinned against the
+    reference's kernels compiled into oracle/_ref."""
+    buf_x = _f64(buf_x).ravel()
+    buf_y = np.zeros_like(buf_x) if out is None else out
+    st = lib().orc_dltc_transpose(buf_x, _i32(x), buf_y, _i32(y), len(x), _i32(wx),
+                                  1 if q2_literal else 0)
+    return st, buf_y
+
+
+# --------------------------------------------------------------------------
+# block grid / symmetry / big matrix
+# --------------------------------------------------------------------------
+def block_rank(addr, grid):
+    return int(lib().orc_block_rank(_i32(addr), len(grid), _i32(grid)))
+
+
+def block_addr(rank, grid):
+    a = np.zeros(len(grid), dtype=np.int32)
+    lib().orc_block_addr(int(rank), len(grid), _i32(grid), a)
+    return a
+
+
+def symm_permutation(addr, sg):
+    s = np.zeros(len(addr), dtype=np.int32)
+    lib().orc_symm_permutation(_i32(addr), _i32(sg), len(addr), s)
+    return s
+
+
+def compose_p_maps(map1, map2):
+    o = np.zeros(len(map1), dtype=np.int32)
+    lib().orc_compose_p_maps(_i32(map1), _i32(map2), len(map1), o)
+    return o
+
+
+def big_matrix_order(vgrid, p_map, permuted_addr, nrow_dims):
+    """transpose_helper.cc:20-171 tile placement.  Returns (order, nrows, ncols)
+    or raises ValueError where the reference asserts (:126)."""
+    pa = _i32(permuted_addr).reshape(-1, len(vgrid))
+    n = pa.shape[0]
+    order = np.zeros(max(n, 1), dtype=np.int64)
+    nr, nc = _i64(), _i64()
+    st = lib().orc_big_matrix_order(len(vgrid), _i32(vgrid), _i32(p_map), _i32(pa.ravel()),
+                                    n, nrow_dims, order, C.byref(nr), C.byref(nc))
+    if st != 0:
+        raise ValueError("rows of the big matrix hold different column counts")
+    return order[: nr.value * nc.value], nr.value, nc.value
+
+
+def big_matrix_pack(tiles, order, nrows, ncols, rpb, cpb):
+    tiles = _f64(tiles).ravel()
+    big = np.zeros(nrows * rpb * ncols * cpb, dtype=np.float64)
+    lib().orc_big_matrix_pack(tiles, big, _np64(order), nrows, ncols, rpb, cpb, rpb * cpb, 0)
+    return big.reshape(nrows * rpb, ncols * cpb)
+
+
+def big_matrix_unpack(big, order, nrows, ncols, rpb, cpb, ntiles):
+    big = _f64(big).ravel()
+    tiles = np.zeros(ntiles * rpb * cpb, dtype=np.float64)
+    lib().orc_big_matrix_pack(tiles, big, _np64(order), nrows, ncols, rpb, cpb, rpb * cpb, 1)
+    return tiles
+
+
+def _np64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+# --------------------------------------------------------------------------
+# GEMM
+# --------------------------------------------------------------------------
+def dgemm(at, bt, m, n, k, alpha, A, lda, B, ldb, Cm, ldc):
+    """localcontract.cc:3-21 / compute.cc:3-36 intended math, in place on Cm.
+    Pinned against numpy float64 matmul."""
+    lib().orc_dgemm(int(at), int(bt), m, n, k, float(alpha), _f64(A).ravel(), lda,
+                    _f64(B).ravel(), ldb, Cm, ldc)
+    return Cm
+
+
+def dgemm_nt_blocked(m, n, k, alpha, A, lda, B, ldb, Cm, ldc, nthreads=0):
+    lib().orc_dgemm_nt_blocked(m, n, k, float(alpha), A, lda, B, ldb, Cm, ldc, nthreads)
+    return Cm
+
+
+# --------------------------------------------------------------------------
+# checkers, drand48, CC closed forms
+# --------------------------------------------------------------------------
+def checkM(A, B, O, V):
+    sC = np.zeros(V * V * O * O)
+    lib().orc_checkM(_f64(A).ravel(), _f64(B).ravel(), sC, O, V)
+    return sC
+
+
+def checkS(A, O, V):
+    sC = np.zeros(V * V * O * O)
+    lib().orc_checkS(_f64(A).ravel(), sC, O, V)
+    return sC
+
+
+class Drand48:
+    """glibc drand48 restated (default seed 0x1234ABCD330E)."""
+
+    def __init__(self, state=0x1234ABCD330E):
+        self.state = C.c_uint64(state)
+
+    def fill(self, n):
+        out = np.empty(n, dtype=np.float64)
+        lib().orc_drand48_fill(out, n, C.byref(self.state))
+        return out
+
+
+def denom_update(R, eo, ev):
+    o, v = len(eo), len(ev)
+    T = np.empty(v * v * o * o)
+    lib().orc_denom_update(_f64(R).ravel(), T, _f64(eo), _f64(ev), o, v)
+    return T
+
+
+def energy(Vijab, Tabij, o, v):
+    return float(lib().orc_energy(_f64(Vijab).ravel(), _f64(Tabij).ravel(), o, v))
+
+
+def norm2(R):
+    R = _f64(R).ravel()
+    return float(lib().orc_norm2(R, R.size))
+
+
+def block_pairs(m, n, k, aoff, boff, coff, alpha, A, B, Cbuf):
+    lib().orc_block_pairs(len(m), _np64(m), _np64(n), _np64(k), _np64(aoff), _np64(boff),
+                          _np64(coff), _f64(alpha), _f64(A).ravel(), _f64(B).ravel(), Cbuf)
+    return Cbuf
+
+
+# --------------------------------------------------------------------------
+# index bookkeeping restated in Python
+# --------------------------------------------------------------------------
+def permutation_maps(a_idx, b_idx, c_idx, block_range):
+    """contraction.cc:2553-2572 + 164-282.
+
+    a_idx/b_idx/c_idx: lists of index names (e.g. ['i','k','a','c']).
+    block_range: dict name -> tile extent.  Returns dict with p_map_A/B/C,
+    inv_p_map_C, inverse_block_range_C, n_a, n_b, n_k, contr_list.
+    Pinned by the closed-form checks in tests (maps are permutations,
+    externals keep their relative order, contraction pairs follow contr_list).
+    """
+    contr = [(i, j) for i, x in enumerate(a_idx) for j, y in enumerate(b_idx) if x == y]
+    ca = {i for i, _ in contr}
+    cb = {j for _, j in contr}
+    p_a = [0] * len(a_idx)
+    p_b = [0] * len(b_idx)
+    p_c = [0] * len(c_idx)
+    inv_c = [0] * len(c_idx)
+    inv_br = [0] * len(c_idx)
+    n_a = n_b = nk_a = nk_b = 1
+    ea = 0
+    for i, x in enumerate(a_idx):
+        if i not in ca:
+            n_a *= block_range[x]
+            p_a[i] = ea
+            ci = c_idx.index(x)
+            p_c[ci] = ea
+            inv_br[ea] = block_range[x]
+            inv_c[ea] = ci
+            ea += 1
+        else:
+            nk_a *= block_range[x]
+    eb = 0
+    for j, x in enumerate(b_idx):
+        if j not in cb:
+            n_b *= block_range[x]
+            p_b[j] = eb
+            ci = c_idx.index(x)
+            p_c[ci] = ea + eb
+            inv_br[ea + eb] = block_range[x]
+            inv_c[ea + eb] = ci
+            eb += 1
+        else:
+            nk_b *= block_range[x]
+    assert nk_a == nk_b
+    for t, (i, j) in enumerate(contr):
+        p_a[i] = ea + t
+        p_b[j] = eb + t
+    return dict(p_map_A=p_a, p_map_B=p_b, p_map_C=p_c, inv_p_map_C=inv_c,
+                inverse_block_range_C=inv_br, n_a=n_a, n_b=n_b, n_k=nk_a,
+                contr_list=contr)
+
+
+def dltc_index_split(ida, idb, nidx=24):
+    """DLTC/expr.cc:13-67: m (A only), n (B only), k (both), each in index
+    enum order; at = m+k, bt = k+n, ct = m+n."""
+    tab = [0] * nidx
+    for x in ida:
+        tab[x] = 1
+    for x in idb:
+        tab[x] += 2
+    m = [i for i in range(nidx) if tab[i] == 1]
+    n = [i for i in range(nidx) if tab[i] == 2]
+    k = [i for i in range(nidx) if tab[i] == 3]
+    return dict(m=m, n=n, k=k, at=m + k, bt=k + n, ct=m + n)
+
+
+def rrr_tile_addresses(tensor_str, vgrid):
+    """tensor.cc:652-693 generate_data order for one rank (pgrid all 1):
+    odometer over tile addresses, last fastest, bounded per get_bound
+    (tensor.cc:761-783), kept iff isValid (tensor.cc:617-650)."""
+    dims = len(tensor_str)
+    group = []
+    for ch in tensor_str:
+        group.append(0 if ch in "aA" else 1 if ch in "bB" else 3)
+    out = []
+
+    def bound(d, cur):
+        ch = tensor_str[d]
+        if ch in "cCdD" or d == 0:
+            return vgrid[d]
+        for i in range(d - 1, -1, -1):
+            if (ch in "aA" and tensor_str[i] in "aA") or (ch in "bB" and tensor_str[i] in "bB"):
+                return cur[i] + 1
+        return vgrid[d]
+
+    def valid(cur):
+        for i in range(dims):
+            for j in range(i + 1, dims):
+                if cur[i] < cur[j] and group[i] == group[j] and group[i] != 3:
+                    return False
+        return all(cur[i] < vgrid[i] for i in range(dims))
+
+    cur = [0] * dims
+
+    def rec(d):
+        for i in range(bound(d, cur)):
+            cur[d] = i
+            if d == dims - 1:
+                if valid(cur):
+                    out.append(list(cur))
+            else:
+                rec(d + 1)
+
+    rec(0)
+    return np.array(out, dtype=np.int32).reshape(-1, dims), group
+
+
+# --------------------------------------------------------------------------
+# reference kernels compiled from source (oracle/_ref)
+# --------------------------------------------------------------------------
+DLTC_4D_NAMES = [
+    "ijkl", "ijlk", "ikjl", "iklj", "iljk", "ilkj", "jikl", "jilk", "jkil", "jkli",
+    "jlik", "jlki", "kijl", "kilj", "kjil", "kjli", "klij", "klji", "lijk", "likj",
+    "ljik", "ljki", "lkij", "lkji",
+]
+
+
+def ref_transp(kind="seq"):
+    """The reference's own DLTC transpose kernels (transp-kernel-{seq,omp}.cc)
+    compiled by oracle/Makefile into oracle/_ref, or None if not built."""
+    if kind in _REF:
+        return _REF[kind]
+    path = os.path.join(_HERE, "_ref", f"libref_transp_{kind}.so")
+    if not os.path.exists(path):
+        _REF[kind] = None
+        return None
+    L = C.CDLL(path)
+    fns = {}
+    for nm in ["ij", "ji"]:
+        f = getattr(L, f"_Z{len('transpose_' + nm)}transpose_{nm}PKdPdii")
+        f.argtypes = [_dp, _dp, C.c_int, C.c_int]
+        fns[nm] = f
+    for nm in DLTC_4D_NAMES:
+        f = getattr(L, f"_Z{len('transpose_' + nm)}transpose_{nm}PKdPdiiii")
+        f.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        fns[nm] = f
+    _REF[kind] = fns
+    return fns

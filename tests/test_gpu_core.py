@@ -1,0 +1,311 @@
+"""GPU parity of the core kernels against the CPU oracle (bit-exact for
+permutes and packs; normwise relative 1e-12 for FP64 GEMM)."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from dtcf_amd import ops, rrr
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(7)
+GEMM_TOL = 1e-12  # max|C - C_ref| / max|C_ref|, FP64
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(got, ref):
+    ref = np.asarray(ref)
+    return float(np.max(np.abs(np.asarray(got) - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+@pytest.mark.parametrize("w", [[5, 7], [32, 33], [3, 4, 5], [12, 12, 48, 48], [2, 3, 4, 5, 3],
+                               [1, 9, 1, 4], [2, 3, 2, 4, 3, 2]])
+def test_permute_tiles_bit_exact(dev, w):
+    say("permute", w)
+    n = int(np.prod(w))
+    ntiles = 3
+    x = RNG.standard_normal(n * ntiles)
+    xd = torch.from_numpy(x).to(dev)
+    perms = list(itertools.permutations(range(len(w))))
+    if len(perms) > 12:
+        perms = [perms[i] for i in RNG.choice(len(perms), 12, replace=False)]
+    for pm in perms:
+        out = ops.rrr_transpose(xd, list(pm), w, ntiles=ntiles)
+        ref = np.concatenate([oracle.rrr_transpose(x[t * n:(t + 1) * n], pm, w)
+                              for t in range(ntiles)])
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), ref), pm
+
+
+def test_permute_all_dltc_perms_bit_exact(dev):
+    shp = [5, 6, 7, 9]
+    n = int(np.prod(shp))
+    x = RNG.standard_normal(n)
+    xd = torch.from_numpy(x).to(dev)
+    for name in oracle.DLTC_4D_NAMES:
+        s = ["ijkl".index(ch) for ch in name]
+        out = ops.permute_tiles(xd, shp, s)
+        assert np.array_equal(out.cpu().numpy(), oracle.permute_by_source(x, s, shp)), name
+
+
+def test_permute_copy_offsets_and_accumulate(dev):
+    # task_sum form: out_tile += coef * perm(in_tile) for a list of tiles
+    w = [4, 6, 5]
+    bs = 120
+    nt = 5
+    x = RNG.standard_normal(bs * nt)
+    y0 = RNG.standard_normal(bs * 8)
+    src = np.array([4, 0, 2, 1, 3]) * bs
+    dst = np.array([7, 1, 0, 5, 2]) * bs
+    s = [2, 0, 1]
+    ext = [w[i] for i in s]
+    istr_in = [30, 5, 1]
+    istr = [istr_in[i] for i in s]
+    ostr = [ext[1] * ext[2], ext[2], 1]
+    xd = torch.from_numpy(x).to(dev)
+    yd = torch.from_numpy(y0.copy()).to(dev)
+    ops.permute_copy(xd, yd, ext, istr, ostr, ntiles=nt,
+                     src_off=torch.from_numpy(src).to(dev), dst_off=torch.from_numpy(dst).to(dev),
+                     alpha=0.5, beta=1.0)
+    ref = y0.copy()
+    for t in range(nt):
+        p = oracle.permute_by_source(x[src[t]:src[t] + bs], s, w)
+        ref[dst[t]:dst[t] + bs] = ref[dst[t]:dst[t] + bs] + 0.5 * p
+    assert np.array_equal(yd.cpu().numpy(), ref)
+
+
+def test_mfma_layout_single_block(dev):
+    # one 16x16x4 product: exposes any mistake in the MFMA lane layout
+    A = np.arange(64, dtype=np.float64).reshape(16, 4) / 7.0
+    B = (np.arange(64, dtype=np.float64).reshape(4, 16) - 30.0) / 3.0
+    Cm = np.zeros((16, 16))
+    Ad, Bd, Cd = (torch.from_numpy(t.copy()).to(dev) for t in (A, B, Cm))
+    ops.gemm(0, 0, 16, 16, 4, 1.0, Ad, 4, Bd, 16, Cd, 16)
+    got = Cd.cpu().numpy()
+    ref = A @ B
+    if relerr(got, ref) > GEMM_TOL:
+        np.set_printoptions(linewidth=200, precision=2)
+        say("got\n", got, "\nref\n", ref)
+    assert relerr(got, ref) <= GEMM_TOL
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("at,bt", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_f64_shapes(dev, cfg, at, bt):
+    ops.set_gemm_config(cfg)
+    try:
+        for (m, n, k) in [(1, 1, 1), (17, 33, 5), (130, 70, 257), (256, 288, 64), (200, 150, 31)]:
+            say("gemm", cfg, at, bt, m, n, k)
+            A = RNG.standard_normal((k, m) if at else (m, k))
+            B = RNG.standard_normal((n, k) if bt else (k, n))
+            C0 = RNG.standard_normal((m, n))
+            Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+            Cd = torch.from_numpy(C0.copy()).to(dev)
+            ops.gemm(at, bt, m, n, k, 0.75, Ad, A.shape[1], Bd, B.shape[1], Cd, n)
+            ref = C0.copy().ravel()
+            oracle.dgemm(at, bt, m, n, k, 0.75, A, A.shape[1], B, B.shape[1], ref, n)
+            assert relerr(Cd.cpu().numpy().ravel(), ref) <= GEMM_TOL, (m, n, k)
+    finally:
+        ops.set_gemm_config(-1)
+
+
+def test_gemm_leading_dims_and_offsets(dev):
+    m, n, k = 45, 38, 29
+    lda, ldb, ldc = 33, 41, 50
+    A = RNG.standard_normal((m, lda))
+    B = RNG.standard_normal((k, ldb))
+    C0 = RNG.standard_normal((m, ldc))
+    Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    Cd = torch.from_numpy(C0.copy()).to(dev)
+    ops.gemm(0, 0, m, n, k, 1.0, Ad, lda, Bd, ldb, Cd, ldc)
+    ref = C0.copy()
+    ref[:, :n] += A[:, :k] @ B[:, :n]
+    got = Cd.cpu().numpy()
+    assert relerr(got, ref) <= GEMM_TOL
+    assert np.array_equal(got[:, n:], C0[:, n:])  # untouched padding
+
+
+def test_gemm_repeat_is_bitwise_deterministic(dev):
+    m = n = k = 384
+    A = torch.from_numpy(RNG.standard_normal((m, k))).to(dev)
+    B = torch.from_numpy(RNG.standard_normal((n, k))).to(dev)
+    outs = []
+    for _ in range(3):
+        Cd = torch.zeros((m, n), dtype=torch.float64, device=dev)
+        ops.gemm(0, 1, m, n, k, 1.0, A, k, B, k, Cd, n)
+        outs.append(Cd.cpu())
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[1], outs[2])
+
+
+def _random_view(rows, cols, rpb, cpb, kind, rng):
+    """A view over a buffer of random tiles with a random intra-tile permutation."""
+    nrb, ncb = rows // rpb, cols // cpb
+    bs = rpb * cpb
+    if kind == "col":       # tile row-major [rpb][cpb]
+        r0 = np.arange(rpb) * cpb
+        c0 = np.arange(cpb)
+    elif kind == "row":     # tile stored transposed [cpb][rpb]
+        r0 = np.arange(rpb)
+        c0 = np.arange(cpb) * rpb
+    else:                   # neither direction contiguous: strided in both
+        r0 = np.arange(rpb) * 2 * cpb
+        c0 = np.arange(cpb) * 2
+        bs = 2 * rpb * cpb
+    order = rng.permutation(nrb * ncb)
+    blk = order * bs
+    rofs = np.tile(r0, nrb)
+    cofs = np.tile(c0, ncb)
+    return rofs, cofs, blk, bs * nrb * ncb
+
+
+def _view_dense(buf, rofs, cofs, blk, rpb, cpb, ncb):
+    rows, cols = len(rofs), len(cofs)
+    R = np.arange(rows) // rpb
+    Cb = np.arange(cols) // cpb
+    idx = rofs[:, None] + cofs[None, :] + blk[R[:, None] * ncb + Cb[None, :]]
+    return buf[idx]
+
+
+@pytest.mark.parametrize("ka,kb,kc", [("col", "col", "col"), ("col", "row", "col"),
+                                      ("row", "col", "row"), ("row", "row", "col"),
+                                      ("str", "col", "col")])
+def test_gemm_view_matches_dense(dev, ka, kb, kc):
+    rng = np.random.default_rng(11)
+    m, n, k = 96, 80, 72
+    rpa, cpa, rpbb, cpbb = 12, 24, 20, 24
+    ra, ca, ba, sa = _random_view(m, k, rpa, cpa, ka, rng)
+    rb, cb, bb, sb = _random_view(n, k, rpbb, cpbb, kb, rng)
+    rc, cc, bc, sc = _random_view(m, n, rpa, rpbb, kc, rng)
+    Abuf, Bbuf, Cbuf = rng.standard_normal(sa), rng.standard_normal(sb), rng.standard_normal(sc)
+    Ad = _view_dense(Abuf, ra, ca, ba, rpa, cpa, k // cpa)
+    Bd = _view_dense(Bbuf, rb, cb, bb, rpbb, cpbb, k // cpbb)
+    Cd0 = _view_dense(Cbuf, rc, cc, bc, rpa, rpbb, n // rpbb)
+    av = ops.View(m, k, ra, ca, ba, rpa, cpa)
+    bv = ops.View(n, k, rb, cb, bb, rpbb, cpbb)
+    cv = ops.View(m, n, rc, cc, bc, rpa, rpbb)
+    assert av.span <= sa and bv.span <= sb and cv.span <= sc
+    Ag, Bg, Cg = (torch.from_numpy(x.copy()).to(dev) for x in (Abuf, Bbuf, Cbuf))
+    ops.gemm_view(m, n, k, -1.25, Ag, av, Bg, bv, Cg, cv)
+    got = _view_dense(Cg.cpu().numpy(), rc, cc, bc, rpa, rpbb, n // rpbb)
+    ref = Cd0 - 1.25 * Ad @ Bd.T
+    assert relerr(got, ref) <= GEMM_TOL
+    # gather / scatter round trip is bit-exact
+    big = torch.empty((m, k), dtype=torch.float64, device=dev)
+    ops.view_gather(Ag, av, big, k)
+    assert np.array_equal(big.cpu().numpy(), Ad)
+    back = torch.zeros_like(Ag)
+    ops.view_scatter(big, k, av, back)
+    touched = np.zeros(sa, bool)
+    idx = (ra[:, None] + ca[None, :] + ba[(np.arange(m) // rpa)[:, None] * (k // cpa)
+                                          + (np.arange(k) // cpa)[None, :]])
+    touched[idx.ravel()] = True
+    assert np.array_equal(back.cpu().numpy()[touched], Abuf[touched])
+
+
+def _c2_like(o, v, vg, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((o, o, v, v)) * 0.05
+    B = rng.standard_normal((o, o, v, v)) * 0.05
+    Cm = rng.standard_normal((o, o, v, v)) * 0.05
+    return A, B, Cm
+
+
+@pytest.mark.parametrize("mode", [rrr.LITERAL, rrr.PACKED, rrr.FUSED])
+def test_local_step_ring_modes(dev, mode):
+    # C[i,j,a,b] += sum_{k,c} A[i,k,a,c] B[k,j,c,b]  (the ring term of config 2, small)
+    o, v = 8, 24
+    vg = [2, 2, 3, 3]
+    A, B, Cm = _c2_like(o, v, vg, 3)
+    tA = rrr.TileTensor.from_dense("cccc", A, vg, dev)
+    tB = rrr.TileTensor.from_dense("cccc", B, vg, dev)
+    tC = rrr.TileTensor.from_dense("cccc", Cm, vg, dev)
+    say("step mode", mode)
+    st = rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=mode)
+    assert st.big_mnk == (o * v, o * v, o * v)
+    st.run(1.0)
+    torch.cuda.synchronize()
+    got = rrr.dense_from_tiles(tC.tiles.cpu().numpy(), Cm.shape, vg, tC.addrs)
+    ref = Cm + np.einsum("ikac,kjcb->ijab", A, B)
+    assert relerr(got, ref) <= GEMM_TOL
+
+
+def test_local_step_ladder_and_transposed_output(dev):
+    # C1 shape (scaled down): R[a,b,i,j] += 0.5 * sum_{cd} T[c,d,i,j] V[a,b,c,d]
+    o, v = 4, 8
+    rng = np.random.default_rng(5)
+    T = rng.standard_normal((v, v, o, o))
+    V = rng.standard_normal((v, v, v, v))
+    R = rng.standard_normal((v, v, o, o))
+    vgT, vgV, vgR = [2, 2, 2, 1], [2, 2, 2, 2], [2, 2, 2, 1]
+    for mode in (rrr.LITERAL, rrr.PACKED, rrr.FUSED):
+        tT = rrr.TileTensor.from_dense("cccc", T, vgT, dev)
+        tV = rrr.TileTensor.from_dense("cccc", V, vgV, dev)
+        tR = rrr.TileTensor.from_dense("cccc", R, vgR, dev)
+        st = rrr.LocalStep(tT, "c,d,i,j", tV, "a,b,c,d", tR, "a,b,i,j", mode=mode)
+        st.run(0.5)
+        got = rrr.dense_from_tiles(tR.tiles.cpu().numpy(), R.shape, vgR, tR.addrs)
+        ref = R + 0.5 * np.einsum("cdij,abcd->abij", T, V)
+        assert relerr(got, ref) <= GEMM_TOL, mode
+
+
+def test_local_step_symmetric_bounce_literal(dev):
+    # A is permutationally symmetric in (i,k) ('aacc').  Only tiles with
+    # addr0 >= addr1 are stored; a bounced copy arrives under the swapped
+    # address carrying the stored tile's data, and the plan must swap its
+    # dims back without a sign (get_symm_permutation, contraction.cc:132-160).
+    o, v = 6, 8
+    vg = [3, 3, 2, 2]
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((o, o, v, v))
+    full = X + X.transpose(1, 0, 2, 3)
+    stored = rrr.tile_addresses("aacc", vg)
+    br = [o // 3, o // 3, v // 2, v // 2]
+    addrs, datas = [], []
+    for a in stored:
+        sl = tuple(slice(a[d] * br[d], (a[d] + 1) * br[d]) for d in range(4))
+        blk = full[sl].ravel()
+        addrs.append(list(a))
+        datas.append(blk)
+        if a[0] > a[1]:
+            addrs.append([a[1], a[0], a[2], a[3]])
+            datas.append(blk)  # same data under the bounced address
+    tA = rrr.TileTensor("aacc", full.shape, vg, dev,
+                        data=torch.from_numpy(np.concatenate(datas)).to(dev), addrs=addrs)
+    B = rng.standard_normal((o, o, v, v))
+    tB = rrr.TileTensor.from_dense("cccc", B, vg, dev)
+    Cm = rng.standard_normal((o, o, v, v))
+    tC = rrr.TileTensor.from_dense("cccc", Cm, vg, dev)
+    st = rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=rrr.LITERAL)
+    st.run(1.0)
+    got = rrr.dense_from_tiles(tC.tiles.cpu().numpy(), Cm.shape, vg, tC.addrs)
+    ref = Cm + np.einsum("ikac,kjcb->ijab", full, B)
+    assert relerr(got, ref) <= GEMM_TOL
+    # the fused modes refuse several bounce classes rather than mis-read them
+    with pytest.raises(Exception):
+        rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=rrr.FUSED)
+
+
+def test_reductions_vs_oracle(dev):
+    o, v = 6, 10
+    eo = -2.0 + 1.5 * RNG.random(o)
+    ev = 0.2 + 2.8 * RNG.random(v)
+    R = RNG.standard_normal(v * v * o * o) * 0.05
+    Vm = RNG.standard_normal(o * o * v * v) * 0.05
+    Rd = torch.from_numpy(R).to(dev)
+    T = ops.denom_update(Rd, torch.from_numpy(eo).to(dev), torch.from_numpy(ev).to(dev))
+    Tref = oracle.denom_update(R, eo, ev)
+    assert np.array_equal(T.cpu().numpy(), Tref)  # IEEE division, same operation order
+    E = ops.energy(torch.from_numpy(Vm).to(dev), T, o, v).item()
+    Eref = oracle.energy(Vm, Tref, o, v)
+    assert abs(E - Eref) <= 1e-13 * abs(Eref)
+    n2 = ops.norm2(Rd).item()
+    assert abs(n2 - oracle.norm2(R)) <= 1e-13 * oracle.norm2(R)
+    E2 = ops.energy(torch.from_numpy(Vm).to(dev), T, o, v).item()
+    assert E2 == E  # deterministic

