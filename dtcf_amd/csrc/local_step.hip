@@ -246,6 +246,10 @@ struct dtcf_step_s {
     // per-class source/destination offsets for the LITERAL permute passes
     std::vector<int64_t *> cls_off;  // device arrays
     bool empty = false;
+    // Optional timing of the GEMM kernel: one event pair per run, recorded on
+    // the stream the GEMM is launched on.
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
 };
 
 static int step_alloc(double **p, int64_t n)
@@ -266,6 +270,10 @@ extern "C" int dtcf_step_destroy(dtcf_step s)
         if (p) (void)hipFree(p);
     for (int64_t *p : s->cls_off)
         if (p) (void)hipFree(p);
+    for (auto &e : s->ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     delete s;
     return DTCF_OK;
 }
@@ -474,22 +482,75 @@ static int permute_operand(dtcf_step s, OperandPlan &P, size_t off_base, bool in
     return DTCF_OK;
 }
 
+extern "C" int dtcf_step_enable_timing(dtcf_step s, int max_runs)
+{
+    DTCF_REQUIRE(s && max_runs >= 0, "step: bad timing request");
+    for (auto &e : s->ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    s->ev.clear();
+    s->ev_used = 0;
+    for (int i = 0; i < max_runs; i++) {
+        hipEvent_t a, b;
+        DTCF_HIP(hipEventCreate(&a));
+        DTCF_HIP(hipEventCreate(&b));
+        s->ev.push_back({a, b});
+    }
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_step_kernel_times(dtcf_step s, float *ms, int max_n, int *n_out)
+{
+    DTCF_REQUIRE(s && n_out, "step: null argument");
+    const int n = (int)std::min<size_t>(s->ev_used, (size_t)std::max(max_n, 0));
+    for (int i = 0; i < n; i++) {
+        DTCF_HIP(hipEventSynchronize(s->ev[i].second));
+        DTCF_HIP(hipEventElapsedTime(&ms[i], s->ev[i].first, s->ev[i].second));
+    }
+    *n_out = n;
+    s->ev_used = 0;
+    return DTCF_OK;
+}
+
+// Records the start event of the next free pair (if timing is enabled).
+static hipEvent_t *timing_pair(dtcf_step s)
+{
+    if (s->ev_used >= s->ev.size()) return nullptr;
+    return &s->ev[s->ev_used].first;
+}
+
+static int gemm_timed(dtcf_step s, hipStream_t st, double alpha)
+{
+    hipEvent_t *e = timing_pair(s);
+    if (e) DTCF_HIP(hipEventRecord(e[0], st));
+    int r;
+    if (s->mode == DTCF_STEP_FUSED)
+        r = dtcf_gemm_view_f64(s->M, s->N, s->K, alpha, s->A.tiles, s->va, s->B.tiles, s->vb,
+                               s->C.tiles, s->vc, 0, st);
+    else
+        r = dtcf_gemm_f64(0, 1, s->M, s->N, s->K, alpha, s->bigA, s->K, s->bigB, s->K, s->bigC,
+                          s->N, st);
+    if (r != DTCF_OK) return r;
+    if (e) {
+        DTCF_HIP(hipEventRecord(s->ev[s->ev_used].second, st));
+        s->ev_used++;
+    }
+    return DTCF_OK;
+}
+
 extern "C" int dtcf_step_run(dtcf_step s, double alpha, void *stream)
 {
     DTCF_REQUIRE(s, "step: null handle");
     if (s->empty) return DTCF_OK;
     hipStream_t st = as_stream(stream);
     int r;
-    if (s->mode == DTCF_STEP_FUSED)
-        return dtcf_gemm_view_f64(s->M, s->N, s->K, alpha, s->A.tiles, s->va, s->B.tiles, s->vb,
-                                  s->C.tiles, s->vc, 0, st);
+    if (s->mode == DTCF_STEP_FUSED) return gemm_timed(s, st, alpha);
     if (s->mode == DTCF_STEP_PACKED) {
         if ((r = dtcf_view_gather_f64(s->C.tiles, s->vc, s->bigC, s->N, st)) != DTCF_OK) return r;
         if ((r = dtcf_view_gather_f64(s->B.tiles, s->vb, s->bigB, s->K, st)) != DTCF_OK) return r;
         if ((r = dtcf_view_gather_f64(s->A.tiles, s->va, s->bigA, s->K, st)) != DTCF_OK) return r;
-        if ((r = dtcf_gemm_f64(0, 1, s->M, s->N, s->K, alpha, s->bigA, s->K, s->bigB, s->K,
-                               s->bigC, s->N, st)) != DTCF_OK)
-            return r;
+        if ((r = gemm_timed(s, st, alpha)) != DTCF_OK) return r;
         return dtcf_view_scatter_f64(s->bigC, s->N, s->vc, s->C.tiles, st);
     }
     // LITERAL: the reference's pass structure.
@@ -500,9 +561,7 @@ extern "C" int dtcf_step_run(dtcf_step s, double alpha, void *stream)
     if ((r = dtcf_view_gather_f64(s->tmp, s->vb, s->bigB, s->K, st)) != DTCF_OK) return r;
     if ((r = permute_operand(s, s->A, baseA, false, st)) != DTCF_OK) return r;
     if ((r = dtcf_view_gather_f64(s->tmp, s->va, s->bigA, s->K, st)) != DTCF_OK) return r;
-    if ((r = dtcf_gemm_f64(0, 1, s->M, s->N, s->K, alpha, s->bigA, s->K, s->bigB, s->K, s->bigC,
-                           s->N, st)) != DTCF_OK)
-        return r;
+    if ((r = gemm_timed(s, st, alpha)) != DTCF_OK) return r;
     if ((r = dtcf_view_scatter_f64(s->bigC, s->N, s->vc, s->tmp, st)) != DTCF_OK) return r;
     return permute_operand(s, s->C, baseC, true, st);
 }

@@ -148,6 +148,15 @@ class LocalStep:
         check(lib().dtcf_step_run(self.h, float(alpha),
                                   C.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
+    def enable_timing(self, max_runs: int):
+        check(lib().dtcf_step_enable_timing(self.h, int(max_runs)))
+
+    def kernel_times_ms(self, max_n: int = 4096):
+        buf = (C.c_float * max_n)()
+        n = C.c_int()
+        check(lib().dtcf_step_kernel_times(self.h, buf, max_n, C.byref(n)))
+        return [float(buf[i]) for i in range(n.value)]
+
     def __del__(self):
         try:
             if self.h:
@@ -169,6 +178,10 @@ def _declare_step():
     L.dtcf_step_info.restype = C.c_int
     L.dtcf_step_destroy.argtypes = [_vp]
     L.dtcf_step_destroy.restype = C.c_int
+    L.dtcf_step_enable_timing.argtypes = [_vp, C.c_int]
+    L.dtcf_step_enable_timing.restype = C.c_int
+    L.dtcf_step_kernel_times.argtypes = [_vp, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]
+    L.dtcf_step_kernel_times.restype = C.c_int
 
 
 try:

@@ -187,6 +187,11 @@ int dtcf_step_run(dtcf_step s, double alpha, void *stream);
  * GEMM sizes M, N, K. */
 int dtcf_step_info(dtcf_step s, int64_t *mnk_tile, int64_t *mnk_big);
 int dtcf_step_destroy(dtcf_step s);
+/* Measurement: record a hipEvent pair around the GEMM kernel of each of the
+ * next `max_runs` runs, on the stream the kernel is launched on;
+ * dtcf_step_kernel_times waits for and returns the durations (ms). */
+int dtcf_step_enable_timing(dtcf_step s, int max_runs);
+int dtcf_step_kernel_times(dtcf_step s, float *ms, int max_n, int *n_out);
 
 /* ------------------------------------------------------------------ */
 /* 4. FP64 reductions and amplitude update (new; no reference          */
