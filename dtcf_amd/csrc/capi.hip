@@ -117,26 +117,52 @@ int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int6
         rpb = rows;
         cpb = cols;
     }
-    int64_t rmax = 0, cmax = 0, bmax = 0;
-    bool reven = true, ceven = true, beven = true;
-    for (int64_t r = 0; r < rows; r++) {
-        DTCF_REQUIRE(rofs[r] >= 0, "view: negative row offset");
-        rmax = std::max(rmax, rofs[r]);
-        reven &= (rofs[r] % 2 == 0);
-    }
-    for (int64_t c = 0; c < cols; c++) {
-        DTCF_REQUIRE(cofs[c] >= 0, "view: negative column offset");
-        cmax = std::max(cmax, cofs[c]);
-        ceven &= (cofs[c] % 2 == 0);
-    }
+    int64_t bmax = 0;
+    for (int64_t r = 0; r < rows; r++) DTCF_REQUIRE(rofs[r] >= 0, "view: negative row offset");
+    for (int64_t c = 0; c < cols; c++) DTCF_REQUIRE(cofs[c] >= 0, "view: negative column offset");
     if (blk)
         for (int64_t b = 0; b < nrb * ncb; b++) {
             DTCF_REQUIRE(blk[b] >= 0, "view: negative block offset");
             bmax = std::max(bmax, blk[b]);
-            beven &= (blk[b] % 2 == 0);
         }
+    // Largest addressable element: rows and columns are independent except
+    // through the block table.
+    int64_t span = 0;
+    {
+        int64_t rm = 0, cm = 0;
+        for (int64_t r = 0; r < rows; r++) rm = std::max(rm, rofs[r]);
+        for (int64_t c = 0; c < cols; c++) cm = std::max(cm, cofs[c]);
+        span = rm + cm + bmax + 1;
+    }
+    // A block table that is a sum of a row-block part and a column-block part
+    // (every dense tile grid) is folded into the row and column tables, so
+    // the kernels read rofs[r] + cofs[c] without the per-element block lookup.
+    std::vector<int64_t> rf, cf;
+    if (blk) {
+        bool sep = true;
+        for (int64_t R = 0; sep && R < nrb; R++)
+            for (int64_t C = 0; sep && C < ncb; C++)
+                sep = blk[R * ncb + C] == blk[R * ncb] + blk[C] - blk[0];
+        if (sep) {
+            rf.assign(rofs, rofs + rows);
+            cf.assign(cofs, cofs + cols);
+            for (int64_t r = 0; r < rows; r++) rf[r] += blk[(r / rpb) * ncb];
+            for (int64_t c = 0; c < cols; c++) cf[c] += blk[c / cpb] - blk[0];
+            rofs = rf.data();
+            cofs = cf.data();
+            blk = nullptr;
+            nrb = ncb = 1;
+            rpb = rows;
+            cpb = cols;
+        }
+    }
     // Pairs (c, c+1) for even c are adjacent inside one block, and the pair
     // start is even (16-byte aligned for an aligned buffer).
+    bool reven = true, ceven = true, beven = true;
+    for (int64_t r = 0; r < rows; r++) reven &= (rofs[r] % 2 == 0);
+    for (int64_t c = 0; c < cols; c++) ceven &= (cofs[c] % 2 == 0);
+    if (blk)
+        for (int64_t b = 0; b < nrb * ncb; b++) beven &= (blk[b] % 2 == 0);
     bool colpairs = beven && reven && (cols % 2 == 0) && (cpb % 2 == 0);
     for (int64_t c = 0; colpairs && c + 1 < cols; c += 2)
         colpairs = (cofs[c + 1] == cofs[c] + 1) && (cofs[c] % 2 == 0);
@@ -151,7 +177,7 @@ int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int6
     v->cpb = cpb;
     v->ncb = ncb;
     v->nblk = blk ? nrb * ncb : 0;
-    v->span = rmax + cmax + bmax + 1;
+    v->span = span;
     v->colpairs = colpairs;
     v->rowpairs = rowpairs;
     hipError_t e = hipMalloc(&v->rofs, rows * sizeof(int64_t));

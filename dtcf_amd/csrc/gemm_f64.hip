@@ -10,17 +10,26 @@
 // Tiling: a 256-thread workgroup (4 wave64) owns a BM x BN block of C; the
 // waves form a (BM/WM) x (BN/WN) grid and each wave owns WM x WN as
 // (WM/16) x (WN/16) MFMA 16x16 accumulators (4 doubles per lane each).  K is
-// walked in steps of BK=16 doubles through a double-buffered LDS stage with
-// register prefetch of the next step, one barrier per step.  LDS rows are
-// padded (stride 18 doubles for k-contiguous operands, 17 for row-contiguous
-// ones) so both the global->LDS stores and the fragment reads are free of
-// bank conflicts.  Workgroup ids are remapped so that consecutive C tiles of
-// a row group land on the same XCD (its own L2).
+// walked in steps of BK=16 through a double-buffered LDS stage with register
+// prefetch of the next step and one barrier per step.
+//
+// LDS layout ("fragment-major"): an operand tile of ROWS x 16 is stored as
+// [row/16][s][lane] with lane = fk*16 + row%16 and k = fk*4 + s (s, fk in
+// 0..3), i.e. inside a K step the MFMA k-slices are taken in the order
+// {s, s+4, s+8, s+12}.  The same permutation is used for A and B, so the sum
+// over k is unchanged, and every fragment read of a wave is one contiguous
+// 512-byte span: bank-conflict free whatever lane grouping the LDS uses.  The
+// global->LDS stores are arranged so each 32-lane half of a wave writes 256
+// contiguous bytes.
 //
 // MFMA operand layout (16x16x4 f64, wave64): lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; the result holds D[l/16 + 4*r][l%16] in element r (checked
 // on the device by tests/test_gpu_core.py::test_mfma_layout_single_block).
+//
+// Workgroup ids are remapped so consecutive C tiles of a row group land on
+// the same XCD (its own L2), then swizzled in groups of 8 tile rows.
 #include <algorithm>
+#include <map>
 #include <mutex>
 
 #include "common.hpp"
@@ -34,11 +43,6 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 
 constexpr int BK = 16;
 constexpr int NT = 256;
-
-template <int MODE>
-struct LdsStride {
-    static constexpr int v = (MODE == kDK || MODE == kVK) ? 18 : 17;
-};
 
 // Element offset of (r, c) in an operand view.
 template <int MODE>
@@ -57,76 +61,152 @@ __device__ inline int64_t op_addr(const OpArgs &o, int64_t r, int64_t c)
     }
 }
 
-// Loads one BROWS x BK operand tile (rows of the view x k) into registers
-// and stores it to LDS laid out [row][k] with the padded stride.
-template <int BROWS, int MODE>
+// Loads one ROWS x BK operand tile into registers (a pair of doubles per
+// pass) and stores it to LDS in the fragment-major layout.
+//
+// k-contiguous operands (kDK, kVK): in pass i, wave w, lane l the pair is
+//   row  = 16*g + l%16,  g = (4i + w) / 2
+//   fk   = (l/16 & 1) + 2*(w & 1),  sp = l/32,  k = 4*fk + 2*sp + {0, 1}
+// so each row contributes 64 contiguous bytes per pass and each 32-lane half
+// of the wave writes 256 contiguous LDS bytes.
+// row-contiguous operands (kDMN, kVMN): the pair is two adjacent rows
+//   r = 2*(l%8), fk = (l/8)%4, s = l/32 + 2*((4i+w) & 1), g = (4i + w) / 2
+// so 8 lanes read 128 contiguous bytes of one k row and the wave writes 1 KiB
+// of contiguous LDS.
+template <int ROWS, int MODE>
 struct TileLoader {
     static constexpr bool KC = (MODE == kDK || MODE == kVK);
-    static constexpr int PAIRS = BROWS * BK / 2;
-    static constexpr int NP = (PAIRS + NT - 1) / NT;
-    static constexpr int S = LdsStride<MODE>::v;
+    static constexpr int NP = ROWS / 32;
+    static_assert(ROWS % 32 == 0, "tile rows must be a multiple of 32");
     d2 reg[NP];
 
-    __device__ inline void load(const OpArgs &o, const double *__restrict__ base, int64_t row0,
-                                int64_t k0, int64_t rows, int64_t K, int tid)
+    __device__ static inline void coords(int i, int w, int l, int &row, int &k0, int &k1,
+                                         int &lds0, int &lds1)
+    {
+        const int u = 4 * i + w, g = u >> 1;
+        if constexpr (KC) {
+            const int r = l & 15, fk = ((l >> 4) & 1) + 2 * (w & 1), sp = l >> 5;
+            row = 16 * g + r;
+            k0 = 4 * fk + 2 * sp;
+            k1 = k0 + 1;
+            lds0 = (g * 4 + 2 * sp) * 64 + fk * 16 + r;
+            lds1 = lds0 + 64;
+        } else {
+            const int r = 2 * (l & 7), fk = (l >> 3) & 3, s = (l >> 5) + 2 * (u & 1);
+            row = 16 * g + r;
+            k0 = k1 = 4 * fk + s;
+            lds0 = (g * 4 + s) * 64 + fk * 16 + r;
+            lds1 = lds0 + 1;
+        }
+    }
+
+    // Generic path with bounds checks (edges, K tails, offset tables).
+    __device__ inline void load_checked(const OpArgs &o, const double *__restrict__ base,
+                                        int64_t row0, int64_t kb, int64_t rows, int64_t K, int w,
+                                        int l)
     {
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            const int p = tid + NT * i;
+            int row, k0, k1, l0, l1;
+            coords(i, w, l, row, k0, k1, l0, l1);
             d2 v = {0.0, 0.0};
-            if (p < PAIRS) {
-                int64_t r0, c0, r1, c1;
-                if constexpr (KC) {
-                    const int row = p / (BK / 2), kp = (p % (BK / 2)) * 2;
-                    r0 = r1 = row0 + row;
-                    c0 = k0 + kp;
-                    c1 = c0 + 1;
+            int64_t r0 = row0 + row, r1 = KC ? r0 : r0 + 1;
+            int64_t c0 = kb + k0, c1 = kb + k1;
+            const bool ok0 = r0 < rows && c0 < K;
+            const bool ok1 = r1 < rows && c1 < K;
+            if (ok0) {
+                const int64_t a0 = op_addr<MODE>(o, r0, c0);
+                if (ok1 && o.vec2) {
+                    v = *reinterpret_cast<const d2 *>(base + a0);
                 } else {
-                    const int kk = p / (BROWS / 2), rp = (p % (BROWS / 2)) * 2;
-                    c0 = c1 = k0 + kk;
-                    r0 = row0 + rp;
-                    r1 = r0 + 1;
-                }
-                const bool ok0 = r0 < rows && c0 < K;
-                const bool ok1 = r1 < rows && c1 < K;
-                if (ok0) {
-                    const int64_t a0 = op_addr<MODE>(o, r0, c0);
-                    if (ok1 && o.vec2) {
-                        v = *reinterpret_cast<const d2 *>(base + a0);
-                    } else {
-                        v.x = base[a0];
-                        if (ok1) v.y = base[op_addr<MODE>(o, r1, c1)];
-                    }
+                    v.x = base[a0];
+                    if (ok1) v.y = base[op_addr<MODE>(o, r1, c1)];
                 }
             }
             reg[i] = v;
         }
     }
 
-    __device__ inline void store(double *lds, int tid) const
+    __device__ inline void store(double *lds, int w, int l) const
     {
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            const int p = tid + NT * i;
-            if (p < PAIRS) {
-                if constexpr (KC) {
-                    const int row = p / (BK / 2), kp = (p % (BK / 2)) * 2;
-                    *reinterpret_cast<d2 *>(lds + row * S + kp) = reg[i];
-                } else {
-                    const int kk = p / (BROWS / 2), rp = (p % (BROWS / 2)) * 2;
-                    lds[rp * S + kk] = reg[i].x;
-                    lds[(rp + 1) * S + kk] = reg[i].y;
-                }
+            int row, k0, k1, l0, l1;
+            coords(i, w, l, row, k0, k1, l0, l1);
+            if constexpr (KC) {
+                lds[l0] = reg[i].x;
+                lds[l1] = reg[i].y;
+            } else {
+                *reinterpret_cast<d2 *>(lds + l0) = reg[i];
             }
         }
     }
 };
 
-// Workgroups per CU the register budget is sized for: two for tiles whose
-// accumulators fit 256 VGPRs, one (with AGPRs for the accumulators) above.
+// Interior fast path for dense operands: per-thread pointers advanced by BK
+// (k-contiguous) or BK*ld (row-contiguous) each step, no bounds checks.
+template <int ROWS, int MODE>
+struct FastPtrs {
+    static constexpr int NP = ROWS / 32;
+    const double *p[NP];
+    int64_t step;
+
+    __device__ inline void init(const OpArgs &o, const double *base, int64_t row0, int w, int l)
+    {
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            int row, k0, k1, l0, l1;
+            TileLoader<ROWS, MODE>::coords(i, w, l, row, k0, k1, l0, l1);
+            p[i] = base + op_addr<MODE>(o, row0 + row, k0);
+        }
+        step = (MODE == kDK) ? BK : BK * o.ld;
+    }
+
+    __device__ inline void load(TileLoader<ROWS, MODE> &t, int64_t kt) const
+    {
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+            t.reg[i] = *reinterpret_cast<const d2 *>(p[i] + kt * step);
+    }
+};
+
+// Interior fast path for view operands whose block table folded away:
+// (r, c) at rofs[r] + cofs[c].  Row offsets are read once; the column
+// offsets of the K step after next are fetched while the next step's data
+// loads are in flight, so no data load waits on a table load.
+template <int ROWS, int MODE>
+struct ViewFast {
+    static constexpr int NP = ROWS / 32;
+    int64_t rb[NP], cn[NP];
+    int kk[NP];
+
+    __device__ inline void init(const OpArgs &o, int64_t row0, int w, int l)
+    {
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            int row, k0, k1, l0, l1;
+            TileLoader<ROWS, MODE>::coords(i, w, l, row, k0, k1, l0, l1);
+            rb[i] = o.rofs[row0 + row];
+            kk[i] = k0;
+        }
+    }
+    __device__ inline void fetch_cols(const OpArgs &o, int64_t kb)
+    {
+#pragma unroll
+        for (int i = 0; i < NP; i++) cn[i] = o.cofs[kb + kk[i]];
+    }
+    __device__ inline void load(TileLoader<ROWS, MODE> &t, const double *base) const
+    {
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+            t.reg[i] = *reinterpret_cast<const d2 *>(base + rb[i] + cn[i]);
+    }
+};
+
 template <int BM, int BN>
 struct Occ {
-    static constexpr int v = (BM * BN <= 128 * 128) ? 2 : 1;
+    // Two workgroups per CU when the accumulators leave room in 256 VGPRs.
+    static constexpr int v = (BM * BN <= 192 * 96) ? 2 : 1;
 };
 
 template <int BM, int BN, int WM, int WN, int AMODE, int BMODE, bool CVIEW>
@@ -134,27 +214,44 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
 {
     static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
     constexpr int FM = WM / 16, FN = WN / 16;
-    constexpr int SA = LdsStride<AMODE>::v, SB = LdsStride<BMODE>::v;
-    constexpr int LDS_A = BM * SA, LDS_B = BN * SB;
-    extern __shared__ double smem[];
-    // Stage s occupies [s*(LDS_A+LDS_B), ...): A tile first, then B tile.
+    constexpr int LDS_A = BM * BK, LDS_B = BN * BK;
     constexpr int STAGE = LDS_A + LDS_B;
+    extern __shared__ double smem[];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave / (BN / WN), wn = wave % (BN / WN);
 
-    // XCD-aware remap then grouped (GROUP_M tile rows) swizzle.
-    uint32_t pid = blockIdx.x;
-    const uint32_t T = g.tiles_m * g.tiles_n;
-    if ((T & 7u) == 0) pid = (pid & 7u) * (T >> 3) + (pid >> 3);
+    // Logical tile q -> (tm, tn) through a grouped (8 tile rows) swizzle.
+    // Workgroups below t_full own whole tiles; their ids are first remapped
+    // so consecutive tiles land on one XCD (its own L2).  Workgroups above
+    // own one K slice of a remainder tile.
+    const uint32_t bid = blockIdx.x;
+    uint32_t q;
+    int64_t kt0 = 0, kt1 = 0;
+    const int64_t K = g.k;
+    const int64_t nk_all = (K + BK - 1) / BK;
+    bool split = false;
+    uint32_t sidx = 0;
+    if (bid < g.t_full) {
+        q = bid;
+        if ((g.t_full & 7u) == 0) q = (bid & 7u) * (g.t_full >> 3) + (bid >> 3);
+        kt1 = nk_all;
+    } else {
+        split = true;
+        sidx = bid - g.t_full;
+        q = g.t_full + sidx / g.sk;
+        const uint32_t sl = sidx % g.sk;
+        kt0 = (int64_t)sl * g.kchunk;
+        kt1 = min(nk_all, kt0 + g.kchunk);
+    }
     const uint32_t GROUP = 8;
     const uint32_t width = GROUP * g.tiles_n;
-    const uint32_t gid = pid / width;
+    const uint32_t gid = q / width;
     const uint32_t first = gid * GROUP;
     const uint32_t gsz = min(g.tiles_m - first, GROUP);
-    const uint32_t tm = first + (pid % width) % gsz;
-    const uint32_t tn = (pid % width) / gsz;
+    const uint32_t tm = first + (q % width) % gsz;
+    const uint32_t tn = (q % width) / gsz;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
 
     TileLoader<BM, AMODE> la;
@@ -165,42 +262,101 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
 #pragma unroll
         for (int j = 0; j < FN; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
-    const int64_t K = g.k;
-    const int64_t nk = (K + BK - 1) / BK;
-    la.load(g.a, g.A, m0, 0, g.m, K, tid);
-    lb.load(g.b, g.B, n0, 0, g.n, K, tid);
-    la.store(smem, tid);
-    lb.store(smem + LDS_A, tid);
+    constexpr bool DENSE = (AMODE <= kDMN) && (BMODE <= kDMN);
+    constexpr bool VIEWS = (AMODE >= kVK) && (BMODE >= kVK);
+    const bool interior = g.a.vec2 && g.b.vec2 && (K % BK == 0) && (m0 + BM <= g.m) &&
+                          (n0 + BN <= g.n);
+    // Uniform per workgroup: every load of this tile is in range and aligned.
+    const bool fast = DENSE && interior;
+    const bool vfast = VIEWS && interior && !g.a.blk && !g.b.blk;
+    FastPtrs<BM, AMODE> fa;
+    FastPtrs<BN, BMODE> fb;
+    ViewFast<BM, AMODE> va;
+    ViewFast<BN, BMODE> vb;
+    if (kt0 < kt1) {
+        if (fast) {
+            fa.init(g.a, g.A, m0, wave, lane);
+            fb.init(g.b, g.B, n0, wave, lane);
+            fa.load(la, kt0);
+            fb.load(lb, kt0);
+        } else if (vfast) {
+            va.init(g.a, m0, wave, lane);
+            vb.init(g.b, n0, wave, lane);
+            va.fetch_cols(g.a, kt0 * BK);
+            vb.fetch_cols(g.b, kt0 * BK);
+            va.load(la, g.A);
+            vb.load(lb, g.B);
+            if (kt0 + 1 < kt1) {
+                va.fetch_cols(g.a, (kt0 + 1) * BK);
+                vb.fetch_cols(g.b, (kt0 + 1) * BK);
+            }
+        } else {
+            la.load_checked(g.a, g.A, m0, kt0 * BK, g.m, K, wave, lane);
+            lb.load_checked(g.b, g.B, n0, kt0 * BK, g.n, K, wave, lane);
+        }
+        la.store(smem, wave, lane);
+        lb.store(smem + LDS_A, wave, lane);
+    }
     __syncthreads();
 
-    const int fr = lane & 15, fk = lane >> 4;
-    for (int64_t kt = 0; kt < nk; kt++) {
-        const int cur = (int)(kt & 1);
-        const bool more = kt + 1 < nk;
+    const int gA = wm * (WM / 16), gB = wn * (WN / 16);
+    for (int64_t kt = kt0; kt < kt1; kt++) {
+        const int cur = (int)((kt - kt0) & 1);
+        const bool more = kt + 1 < kt1;
         if (more) {
-            la.load(g.a, g.A, m0, (kt + 1) * BK, g.m, K, tid);
-            lb.load(g.b, g.B, n0, (kt + 1) * BK, g.n, K, tid);
+            if (fast) {
+                fa.load(la, kt + 1);
+                fb.load(lb, kt + 1);
+            } else if (vfast) {
+                va.load(la, g.A);
+                vb.load(lb, g.B);
+                if (kt + 2 < kt1) {
+                    va.fetch_cols(g.a, (kt + 2) * BK);
+                    vb.fetch_cols(g.b, (kt + 2) * BK);
+                }
+            } else {
+                la.load_checked(g.a, g.A, m0, (kt + 1) * BK, g.m, K, wave, lane);
+                lb.load_checked(g.b, g.B, n0, (kt + 1) * BK, g.n, K, wave, lane);
+            }
         }
-        const double *a_s = smem + cur * STAGE + (wm * WM + fr) * SA + fk;
-        const double *b_s = smem + cur * STAGE + LDS_A + (wn * WN + fr) * SB + fk;
+        const double *a_s = smem + cur * STAGE + gA * 256 + lane;
+        const double *b_s = smem + cur * STAGE + LDS_A + gB * 256 + lane;
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ks++) {
+        for (int s = 0; s < 4; s++) {
             double af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; i++) af[i] = a_s[i * 16 * SA + ks * 4];
+            for (int i = 0; i < FM; i++) af[i] = a_s[i * 256 + s * 64];
 #pragma unroll
-            for (int j = 0; j < FN; j++) bf[j] = b_s[j * 16 * SB + ks * 4];
+            for (int j = 0; j < FN; j++) bf[j] = b_s[j * 256 + s * 64];
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
                 for (int j = 0; j < FN; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
-        if (more) {
-            la.store(smem + (cur ^ 1) * STAGE, tid);
-            lb.store(smem + (cur ^ 1) * STAGE + LDS_A, tid);
+            // The other stage was last read before the previous barrier, so
+            // the next step's tile can be stored halfway through this one's
+            // MFMAs; only the barrier is left for the end of the step.
+            if (s == 1 && more) {
+                la.store(smem + (cur ^ 1) * STAGE, wave, lane);
+                lb.store(smem + (cur ^ 1) * STAGE + LDS_A, wave, lane);
+            }
         }
         __syncthreads();
+    }
+
+    const int fr = lane & 15, fq = lane >> 4;
+    if (split) {
+        // K slice of a remainder tile: raw partial sums to the workspace,
+        // laid out [slice index][BM][BN]; the reduce kernel adds them to C.
+        double *w = g.ws + (int64_t)sidx * BM * BN;
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    w[(wm * WM + i * 16 + fq + 4 * r) * BN + wn * WN + j * 16 + fr] = acc[i][j][r];
+        return;
     }
 
     // Epilogue: C += alpha * acc  (beta = 1, as every reference call site).
@@ -213,7 +369,7 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
             if (col >= g.n) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t row = m0 + wm * WM + i * 16 + fk + 4 * r;
+                const int64_t row = m0 + wm * WM + i * 16 + fq + 4 * r;
                 if (row >= g.m) continue;
                 int64_t addr;
                 if constexpr (CVIEW) addr = op_addr<kVK>(g.c, row, col);
@@ -223,29 +379,119 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
         }
 }
 
-template <int BM, int BN, int WM, int WN>
-constexpr size_t lds_bytes(int am, int bm)
+// Sums the K slices of each remainder tile in slice order and adds
+// alpha * sum to C (deterministic: the order does not depend on timing).
+template <int BM, int BN, bool CVIEW>
+__global__ __launch_bounds__(256) void split_reduce_kernel(GemmArgs g)
 {
-    return (size_t)2 * (BM * ((am == kDK || am == kVK) ? 18 : 17) +
-                        BN * ((bm == kDK || bm == kVK) ? 18 : 17)) *
-           sizeof(double);
+    const uint32_t t = blockIdx.y;  // remainder tile
+    const uint32_t q = g.t_full + t;
+    const uint32_t GROUP = 8;
+    const uint32_t width = GROUP * g.tiles_n;
+    const uint32_t gid = q / width;
+    const uint32_t first = gid * GROUP;
+    const uint32_t gsz = min(g.tiles_m - first, GROUP);
+    const uint32_t tm = first + (q % width) % gsz;
+    const uint32_t tn = (q % width) / gsz;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < BM * BN; e += gridDim.x * 256) {
+        const int64_t row = (int64_t)tm * BM + e / BN, col = (int64_t)tn * BN + e % BN;
+        if (row >= g.m || col >= g.n) continue;
+        double sum = 0.0;
+        const double *w = g.ws + (int64_t)t * g.sk * BM * BN + e;
+        for (uint32_t sl = 0; sl < g.sk; sl++) sum += w[(int64_t)sl * BM * BN];
+        int64_t addr;
+        if constexpr (CVIEW) addr = op_addr<kVK>(g.c, row, col);
+        else addr = row * g.c.ld + col;
+        g.C[addr] += g.alpha * sum;
+    }
+}
+
+// CU count per device, and the tail-split workspace per (device, stream):
+// GEMMs enqueued on different streams never share a workspace; GEMMs on one
+// stream are ordered by it.
+std::mutex g_dev_mu;
+int g_cus[64];
+std::map<std::pair<int, hipStream_t>, std::pair<double *, size_t>> g_ws;
+
+int device_cus(int *cus)
+{
+    int d = 0;
+    DTCF_HIP(hipGetDevice(&d));
+    DTCF_REQUIRE(d >= 0 && d < 64, "gemm: device index out of range");
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_cus[d] == 0) {
+        int c = 0;
+        DTCF_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d));
+        g_cus[d] = c > 0 ? c : 1;
+    }
+    *cus = g_cus[d];
+    return DTCF_OK;
+}
+
+int split_workspace(hipStream_t s, size_t need, double **ws)
+{
+    int d = 0;
+    DTCF_HIP(hipGetDevice(&d));
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    auto &e = g_ws[std::make_pair(d, s)];
+    if (e.second < need) {
+        if (e.first) DTCF_HIP(hipStreamSynchronize(s));  // the old buffer may be in use
+        if (e.first) (void)hipFree(e.first);
+        e.first = nullptr;
+        e.second = 0;
+        DTCF_HIP(hipMalloc(&e.first, need));
+        e.second = need;
+    }
+    *ws = e.first;
+    return DTCF_OK;
 }
 
 template <int BM, int BN, int WM, int WN, int AM, int BMD, bool CV>
-int launch_one(const GemmArgs &g, hipStream_t s)
+int launch_one(GemmArgs g, hipStream_t s)
 {
     auto kern = gemm_f64_kernel<BM, BN, WM, WN, AM, BMD, CV>;
-    const size_t lds = lds_bytes<BM, BN, WM, WN>(AM, BMD);
-    static std::once_flag once;
-    static hipError_t attr_err = hipSuccess;
-    std::call_once(once, [&] {
-        attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    });
-    DTCF_HIP(attr_err);
-    const unsigned grid = g.tiles_m * g.tiles_n;
+    constexpr size_t lds = (size_t)2 * (BM + BN) * BK * sizeof(double);
+    if constexpr (lds > 65536) {
+        static std::once_flag once;
+        static hipError_t attr_err = hipSuccess;
+        std::call_once(once, [&] {
+            attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        });
+        DTCF_HIP(attr_err);
+    }
+    // Tail split: when the last wave of whole tiles would leave most slots
+    // idle, its tiles are cut into K slices that fill one wave instead.
+    int cus = 0;
+    int st = device_cus(&cus);
+    if (st != DTCF_OK) return st;
+    const uint32_t T = g.tiles_m * g.tiles_n;
+    const uint32_t slots = (uint32_t)cus * (uint32_t)Occ<BM, BN>::v;
+    const int64_t nk = (g.k + BK - 1) / BK;
+    const uint32_t rem = T % slots;
+    g.t_full = T;
+    g.sk = 1;
+    g.kchunk = nk;
+    if (T > slots && rem > 0 && rem <= slots / 2 && nk >= 32) {
+        const uint32_t sk = std::min<uint32_t>(slots / rem, (uint32_t)(nk / 8));
+        if (sk >= 2) {
+            double *ws = nullptr;
+            st = split_workspace(s, (size_t)rem * sk * BM * BN * sizeof(double), &ws);
+            if (st != DTCF_OK) return st;
+            g.ws = ws;
+            g.t_full = T - rem;
+            g.sk = sk;
+            g.kchunk = (nk + sk - 1) / sk;
+        }
+    }
+    const unsigned grid = g.t_full + (T - g.t_full) * g.sk;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, g);
     DTCF_HIP(hipGetLastError());
+    if (g.t_full < T) {
+        hipLaunchKernelGGL((split_reduce_kernel<BM, BN, CV>), dim3((BM * BN + 1023) / 1024, T - g.t_full),
+                           dim3(256), 0, s, g);
+        DTCF_HIP(hipGetLastError());
+    }
     return DTCF_OK;
 }
 
@@ -280,11 +526,11 @@ int launch_cfg(GemmArgs g, hipStream_t s)
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s)
 {
     if (g.m <= 0 || g.n <= 0 || g.k <= 0) return DTCF_OK;
-    DTCF_REQUIRE((g.m + 255) / 256 * ((g.n + 255) / 256) < (1ll << 30), "gemm: too many tiles");
+    DTCF_REQUIRE((g.m + 63) / 64 * ((g.n + 63) / 64) < (1ll << 31), "gemm: too many tiles");
     switch (cfg) {
     case kCfg128x128: return launch_cfg<128, 128, 64, 64>(g, s);
     case kCfg192x96: return launch_cfg<192, 96, 96, 48>(g, s);
-    case kCfg128x144: return launch_cfg<128, 144, 32, 144>(g, s);
+    case kCfg96x192: return launch_cfg<96, 192, 48, 96>(g, s);
     case kCfg64x64: return launch_cfg<64, 64, 32, 32>(g, s);
     default: break;
     }

@@ -15,7 +15,7 @@ namespace dtcf {
 enum { kDK = 0, kDMN = 1, kVK = 2, kVMN = 3 };
 
 // Tile configurations (BM x BN), selectable per launch by the autotuner.
-enum { kCfg128x128 = 0, kCfg192x96 = 1, kCfg128x144 = 2, kCfg64x64 = 3, kCfgCount = 4 };
+enum { kCfg128x128 = 0, kCfg192x96 = 1, kCfg96x192 = 2, kCfg64x64 = 3, kCfgCount = 4 };
 
 struct OpArgs {
     int64_t ld;
@@ -37,6 +37,12 @@ struct GemmArgs {
     OpArgs a, b, c;
     int32_t amode, bmode, cview;
     uint32_t tiles_m, tiles_n;
+    // Tail split: workgroups [0, t_full) own whole tiles; the remaining
+    // tiles t_full.. are split over K into `sk` slices of `kchunk` K steps
+    // each, written to `ws` and summed in slice order by a second kernel.
+    uint32_t t_full, sk;
+    int64_t kchunk;
+    double *ws;
 };
 
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);

@@ -128,7 +128,7 @@ int dtcf_view_gather_f64(const double *src, dtcf_view v, double *dst, int64_t ld
 int dtcf_view_scatter_f64(const double *src, int64_t ld, dtcf_view v, double *dst, void *stream);
 
 /* Tile-shape selection for the GEMM kernels: 0 = 128x128, 1 = 192x96,
- * 2 = 128x144, 3 = 64x64, -1 = automatic (autotuned table). */
+ * 2 = 96x192, 3 = 64x64, -1 = automatic (autotuned table). */
 int dtcf_gemm_set_config(int cfg);
 
 /* Host-pointer variant of dtcf_gemm_f64 for the drop-in local_contraction:
