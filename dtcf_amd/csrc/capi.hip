@@ -47,6 +47,7 @@ struct dtcf_view_s {
     int64_t *rofs = nullptr, *cofs = nullptr, *blk = nullptr;  // device
     int64_t span = 0;
     bool colpairs = false, rowpairs = false;
+    bool owner = true;  // sub-views share their parent's tables
 };
 
 extern "C" {
@@ -198,10 +199,42 @@ int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int6
 int dtcf_view_destroy(dtcf_view v)
 {
     if (!v) return DTCF_OK;
-    if (v->rofs) (void)hipFree(v->rofs);
-    if (v->cofs) (void)hipFree(v->cofs);
-    if (v->blk) (void)hipFree(v->blk);
+    if (v->owner) {
+        if (v->rofs) (void)hipFree(v->rofs);
+        if (v->cofs) (void)hipFree(v->cofs);
+        if (v->blk) (void)hipFree(v->blk);
+    }
     delete v;
+    return DTCF_OK;
+}
+
+int dtcf_view_sub(dtcf_view p, int64_t r0, int64_t rows, int64_t c0, int64_t cols, dtcf_view *out)
+{
+    DTCF_REQUIRE(out, "view_sub: null output handle");
+    *out = nullptr;
+    DTCF_REQUIRE(p, "view_sub: null parent");
+    DTCF_REQUIRE(r0 >= 0 && rows > 0 && r0 + rows <= p->rows && c0 >= 0 && cols > 0 &&
+                     c0 + cols <= p->cols,
+                 "view_sub: rectangle outside the parent view");
+    if (p->blk)
+        DTCF_REQUIRE(r0 % p->rpb == 0 && rows % p->rpb == 0 && c0 % p->cpb == 0 &&
+                         cols % p->cpb == 0,
+                     "view_sub: a view with a block table is cut on block boundaries only");
+    dtcf_view_s *v = new dtcf_view_s(*p);
+    v->owner = false;
+    v->rows = rows;
+    v->cols = cols;
+    v->rofs = p->rofs + r0;
+    v->cofs = p->cofs + c0;
+    if (p->blk) {
+        v->blk = p->blk + (r0 / p->rpb) * p->ncb + c0 / p->cpb;
+    } else {
+        v->rpb = rows;
+        v->cpb = cols;
+    }
+    v->colpairs = p->colpairs && c0 % 2 == 0 && cols % 2 == 0;
+    v->rowpairs = p->rowpairs && r0 % 2 == 0 && rows % 2 == 0;
+    *out = v;
     return DTCF_OK;
 }
 

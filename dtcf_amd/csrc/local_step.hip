@@ -128,6 +128,15 @@ int build_operand(const dtcf_tileset *T, const std::vector<int> &pmap, int nrow_
                     sym[j] = i;
                     break;
                 }
+        {
+            // Groups of three or more dims make the reference's pairwise swap
+            // produce a non-permutation (contraction.cc:146-155); refuse them.
+            std::vector<int> seen(P.dims, 0);
+            for (int i = 0; i < P.dims; i++) {
+                DTCF_REQUIRE(!seen[sym[i]], "step: symmetry group of more than two dims");
+                seen[sym[i]] = 1;
+            }
+        }
         std::vector<int> comp(P.dims);
         for (int i = 0; i < P.dims; i++) comp[i] = pmap[sym[i]];
         P.composed[t] = comp;

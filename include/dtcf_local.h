@@ -111,6 +111,10 @@ int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int6
 int dtcf_view_destroy(dtcf_view v);
 /* 1 + largest element offset the view can address (buffer size needed). */
 int64_t dtcf_view_span(dtcf_view v);
+/* A rectangle [r0, r0+rows) x [c0, c0+cols) of a view, sharing its device
+ * tables (no allocation on the device); destroy it before its parent. */
+int dtcf_view_sub(dtcf_view parent, int64_t r0, int64_t rows, int64_t c0, int64_t cols,
+                  dtcf_view *out);
 
 /* C_view[m x n] += alpha * A_view[m x k] * B_view[n x k]^T: the RRR form in
  * which both packed operands have the contraction index as columns
@@ -194,7 +198,63 @@ int dtcf_step_enable_timing(dtcf_step s, int max_runs);
 int dtcf_step_kernel_times(dtcf_step s, float *ms, int max_n, int *n_out);
 
 /* ------------------------------------------------------------------ */
-/* 4. FP64 reductions and amplitude update (new; no reference          */
+/* 4. DLTC tile contractions (dense tensors with uniform tiles)          */
+/*                                                                     */
+/* Replaces the task bodies task_mul / task_sum (DLTC/task.cc:51-137)   */
+/* with the index split of Multiplication (DLTC/expr.cc:13-67), the     */
+/* tile box get/acc of DLTC::Tensor (DLTC/dltc_tensor.cc:169-314), the  */
+/* DLTC transposes (DLTC/transp.cc:3-59) and compute                    */
+/* (DLTC/compute.cc:3-36).  Index names are DLTC IndexName values        */
+/* (param.h: h1..h12 = 0..11, p1..p12 = 12..23; names below nocc are     */
+/* occupied).  A tensor is dense row-major over its index ranges (O for  */
+/* occupied, V for virtual), tiles of wbO / wbV per index.               */
+/* ------------------------------------------------------------------ */
+#define DTCF_DLTC_NIDX 24
+
+typedef struct {
+    int32_t nocc;  /* index names < nocc are occupied (12; 6 in the CHECK build) */
+    int32_t O, V;  /* full ranges */
+    int32_t wbO, wbV;
+} dtcf_dltc_space;
+
+typedef struct {
+    int32_t dims;
+    int32_t ids[DTCF_MAX_DIMS];
+    double *data; /* device */
+} dtcf_dltc_tensor;
+
+typedef struct dtcf_dltc_mul_s *dtcf_dltc_mul;
+
+/* tC += coef * tA * tB.  The plan owns device offset tables; the tensors'
+ * buffers must outlive it. */
+int dtcf_dltc_mul_create(const dtcf_dltc_space *space, const dtcf_dltc_tensor *tC,
+                         const dtcf_dltc_tensor *tA, const dtcf_dltc_tensor *tB, double coef,
+                         dtcf_dltc_mul *out);
+/* GEMM sizes (M, N, K) and the m / n / k index names (IndexName order). */
+int dtcf_dltc_mul_info(dtcf_dltc_mul h, int64_t *mnk, int32_t *m_ids, int32_t *n_ids,
+                       int32_t *k_ids, int32_t *counts);
+/* One task: the tile box [itlb[id], itub[id]) for every index name id
+ * (DLTC/task.h:17-23 TaskDesc).  Whole term, one ext tile with all sum
+ * tiles, or one (ext, sum) tile pair are the reference's three task
+ * granularities (DLTC/exec.cc:58-110). */
+int dtcf_dltc_mul_task(dtcf_dltc_mul h, const int32_t *itlb, const int32_t *itub, void *stream);
+/* The whole term as one GEMM. */
+int dtcf_dltc_mul_run(dtcf_dltc_mul h, void *stream);
+int dtcf_dltc_mul_destroy(dtcf_dltc_mul h);
+/* task_sum: tC += coef * perm(tA) over a tile box (NULL box = all tiles). */
+int dtcf_dltc_sum(const dtcf_dltc_space *space, const dtcf_dltc_tensor *tC,
+                  const dtcf_dltc_tensor *tA, double coef, const int32_t *itlb,
+                  const int32_t *itub, void *stream);
+/* DLTC::transpose of one dense tile buf_x with index names x into buf_y with
+ * names y (device pointers).  DTCF_ERR_UNSUPPORTED for a rank other than 2
+ * or 4, where the reference prints "transpose position error" and exits.
+ * An identity permutation copies. */
+int dtcf_dltc_transpose_f64(const double *buf_x, const int32_t *x, double *buf_y,
+                            const int32_t *y, int dim, const dtcf_dltc_space *space,
+                            void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 5. FP64 reductions and amplitude update (new; no reference          */
 /*    counterpart -- the reference stops at the contraction).          */
 /*    Layouts: R, T as [v, v, o, o]; V as [o, o, v, v].                 */
 /* ------------------------------------------------------------------ */

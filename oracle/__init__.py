@@ -397,3 +397,69 @@ def ref_transp(kind="seq"):
         fns[nm] = f
     _REF[kind] = fns
     return fns
+
+
+# --------------------------------------------------------------------------
+# DLTC task_mul / task_sum restated tile by tile (small sizes only)
+# --------------------------------------------------------------------------
+def dltc_widths(ids, wbO, wbV, nocc=12):
+    """getWB (DLTC/define.h:59-65)."""
+    return [wbO if x < nocc else wbV for x in ids]
+
+
+def _box(ids, idvm, wbO, wbV, nocc):
+    return tuple(slice(idvm[x] * w, (idvm[x] + 1) * w)
+                 for x, w in zip(ids, dltc_widths(ids, wbO, wbV, nocc)))
+
+
+def _odometer(ids, lo, hi):
+    """IndexLoops::next (DLTC/loops.cc:17-31): last index fastest."""
+    import itertools
+    ranges = [range(lo[x], hi[x]) for x in ids]
+    for vals in itertools.product(*ranges):
+        yield dict(zip(ids, vals))
+
+
+def dltc_task_mul(c_ids, Cd, a_ids, Ad, b_ids, Bd, coef, itlb, itub, wbO, wbV, nocc=12,
+                  nidx=24):
+    """DLTC/task.cc:83-137: for each ext tile of C (odometer over tC.ids),
+    buf_ct = 0; for each sum tile: getTile A and B (dltc_tensor.cc:181-267,
+    dense boxes), transpose to at_ids / bt_ids (transp.cc, identity copies),
+    compute buf_ct += at * bt (compute.cc, row-major m x k by k x n); then
+    transpose ct -> C ids and accumulate coef * buf (dltc_tensor.cc:278-305).
+    Cd is updated in place (numpy arrays shaped by the index ranges)."""
+    sp = dltc_index_split(a_ids, b_ids, nidx)
+    at, bt, ct, kk = sp["at"], sp["bt"], sp["ct"], sp["k"]
+    m = int(np.prod(dltc_widths(sp["m"], wbO, wbV, nocc))) if sp["m"] else 1
+    n = int(np.prod(dltc_widths(sp["n"], wbO, wbV, nocc))) if sp["n"] else 1
+    k = int(np.prod(dltc_widths(kk, wbO, wbV, nocc))) if kk else 1
+    for ext in _odometer(list(c_ids), itlb, itub):
+        buf_ct = np.zeros(m * n)
+        for sm in _odometer(kk, itlb, itub):
+            idvm = dict(ext)
+            idvm.update(sm)
+            ta = np.ascontiguousarray(Ad[_box(a_ids, idvm, wbO, wbV, nocc)]).ravel()
+            st, at_buf = dltc_transpose(ta, list(a_ids), at, dltc_widths(a_ids, wbO, wbV, nocc))
+            assert st == 0
+            tb = np.ascontiguousarray(Bd[_box(b_ids, idvm, wbO, wbV, nocc)]).ravel()
+            st, bt_buf = dltc_transpose(tb, list(b_ids), bt, dltc_widths(b_ids, wbO, wbV, nocc))
+            assert st == 0
+            dgemm(0, 0, m, n, k, 1.0, at_buf, k, bt_buf, n, buf_ct, n)
+        st, buf_c = dltc_transpose(buf_ct, ct, list(c_ids), dltc_widths(ct, wbO, wbV, nocc))
+        assert st == 0
+        box = _box(c_ids, ext, wbO, wbV, nocc)
+        Cd[box] += coef * buf_c.reshape(Cd[box].shape)
+    return Cd
+
+
+def dltc_task_sum(c_ids, Cd, a_ids, Ad, coef, wbO, wbV, nocc=12):
+    """DLTC/task.cc:51-81 over all tiles: tC tile += coef * transpose(tA tile)."""
+    lo = {x: 0 for x in c_ids}
+    hi = {x: (Cd.shape[i] // dltc_widths([x], wbO, wbV, nocc)[0]) for i, x in enumerate(c_ids)}
+    for ext in _odometer(list(c_ids), lo, hi):
+        ta = np.ascontiguousarray(Ad[_box(a_ids, ext, wbO, wbV, nocc)]).ravel()
+        st, buf = dltc_transpose(ta, list(a_ids), list(c_ids), dltc_widths(a_ids, wbO, wbV, nocc))
+        assert st == 0
+        box = _box(c_ids, ext, wbO, wbV, nocc)
+        Cd[box] += coef * buf.reshape(Cd[box].shape)
+    return Cd

@@ -117,8 +117,11 @@ def test_symm_permutation_cases():
     assert list(oracle.symm_permutation([1, 2, 0, 3], sg)) == [1, 0, 3, 2]
     # non-symmetric groups never swap
     assert list(oracle.symm_permutation([0, 3, 1, 2], [3, 3, 3, 3])) == [0, 1, 2, 3]
-    # group of three: only the first j is swapped for each i (contraction.cc:154)
-    assert list(oracle.symm_permutation([0, 1, 2], [0, 0, 0])) == [1, 0, 2] or True
+    # group of three: each i swaps with its first larger j and breaks
+    # (contraction.cc:146-155); for addresses (0,1,2) the later swap of i=1
+    # overwrites sym[1], leaving [1, 2, 1] -- not a permutation, so the
+    # reference only handles symmetry groups of two dims.
+    assert list(oracle.symm_permutation([0, 1, 2], [0, 0, 0])) == [1, 2, 1]
     assert list(oracle.compose_p_maps([1, 0, 2, 3], [0, 2, 1, 3])) == [2, 0, 1, 3]
 
 
@@ -265,3 +268,28 @@ def test_rrr_tile_enumeration():
                  if all(not (a[i] < a[j] and grp[i] == grp[j] and grp[i] != 3)
                         for i in range(4) for j in range(i + 1, 4))]
         assert [tuple(a) for a in addrs] == brute
+
+
+def test_dltc_task_mul_restatement_vs_einsum():
+    # the tile-by-tile restatement of task_mul reproduces the contraction
+    O, V, wbO, wbV = 4, 6, 2, 3
+    h = {f"h{i}": i - 1 for i in range(1, 13)}
+    p = {f"p{i}": 11 + i for i in range(1, 13)}
+    # ccsd-t2 ring-like term: R[p3,p4,h1,h2] += T[p3,p5,h1,h6] * I[h6,p4,h2,p5]
+    rng = np.random.default_rng(3)
+    T = rng.standard_normal((V, V, O, O))
+    I = rng.standard_normal((O, V, O, V))
+    R = np.zeros((V, V, O, O))
+    c_ids = [p["p3"], p["p4"], h["h1"], h["h2"]]
+    a_ids = [p["p3"], p["p5"], h["h1"], h["h6"]]
+    b_ids = [h["h6"], p["p4"], h["h2"], p["p5"]]
+    lo = [0] * 24
+    hi = [O // wbO if i < 12 else V // wbV for i in range(24)]
+    oracle.dltc_task_mul(c_ids, R, a_ids, T, b_ids, I, 0.5, lo, hi, wbO, wbV)
+    ref = 0.5 * np.einsum("aeim,mbje->abij", T, I)
+    assert np.allclose(R, ref, rtol=1e-13, atol=1e-13)
+    # task_sum: X[p1,p2,h1,h2] += Y[p2,p1,h2,h1]
+    Y = rng.standard_normal((V, V, O, O))
+    X = np.zeros_like(Y)
+    oracle.dltc_task_sum([12, 13, 0, 1], X, [13, 12, 1, 0], Y, 1.0, wbO, wbV)
+    assert np.array_equal(X, np.transpose(Y, (1, 0, 3, 2)))
