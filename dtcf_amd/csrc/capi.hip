@@ -464,3 +464,43 @@ extern "C" int dtcf_permute_tiles_f64_host(const double *in, double *out, int di
     DTCF_HIP(hipMemcpy(out, dout.p, total * sizeof(double), hipMemcpyDeviceToHost));
     return DTCF_OK;
 }
+
+extern "C" int dtcf_pack_blocks_f64_host(double *tiles, int64_t ntiles, const int64_t *order,
+                                         int64_t nrb, int64_t ncb, int64_t rpb, int64_t cpb,
+                                         double *big, int unpack)
+{
+    DTCF_REQUIRE(ntiles >= 0 && nrb >= 0 && ncb >= 0 && rpb > 0 && cpb > 0, "pack: bad sizes");
+    if (nrb == 0 || ncb == 0) return DTCF_OK;
+    DTCF_REQUIRE(tiles && order && big, "pack: null argument");
+    const int64_t bs = rpb * cpb, rows = nrb * rpb, cols = ncb * cpb;
+    std::vector<int64_t> rofs(rows), cofs(cols), blk(nrb * ncb);
+    for (int64_t r = 0; r < rows; r++) rofs[r] = (r % rpb) * cpb;
+    for (int64_t c = 0; c < cols; c++) cofs[c] = c % cpb;
+    for (int64_t b = 0; b < nrb * ncb; b++) {
+        DTCF_REQUIRE(order[b] >= 0 && order[b] < ntiles, "pack: tile number out of range");
+        blk[b] = order[b] * bs;
+    }
+    dtcf_view v = nullptr;
+    int st = dtcf_view_create(rows, cols, rofs.data(), cofs.data(), blk.data(), rpb, cpb, &v);
+    if (st != DTCF_OK) return st;
+    DevBuf dt, dbig;
+    hipError_t e = hipMalloc(&dt.p, ntiles * bs * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dbig.p, rows * cols * sizeof(double));
+    if (e == hipSuccess)
+        e = unpack ? hipMemcpy(dbig.p, big, rows * cols * sizeof(double), hipMemcpyHostToDevice)
+                   : hipMemcpy(dt.p, tiles, ntiles * bs * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && unpack)  // tiles outside the order keep their values
+        e = hipMemcpy(dt.p, tiles, ntiles * bs * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        dtcf_view_destroy(v);
+        return hip_fail(e, "pack: staging");
+    }
+    st = unpack ? dtcf_view_scatter_f64((const double *)dbig.p, cols, v, (double *)dt.p, nullptr)
+                : dtcf_view_gather_f64((const double *)dt.p, v, (double *)dbig.p, cols, nullptr);
+    dtcf_view_destroy(v);
+    if (st != DTCF_OK) return st;
+    e = unpack ? hipMemcpy(tiles, dt.p, ntiles * bs * sizeof(double), hipMemcpyDeviceToHost)
+               : hipMemcpy(big, dbig.p, rows * cols * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "pack: copy back");
+    return DTCF_OK;
+}

@@ -141,6 +141,14 @@ int dtcf_gemm_f64_host(int transa, int transb, int64_t m, int64_t n, int64_t k,
                        double alpha, const double *A, int64_t lda, const double *B,
                        int64_t ldb, double *C, int64_t ldc);
 
+/* Host-pointer big-matrix pack / unpack (drop-in create_big_matrix /
+ * revert_big_matrix, transpose_helper.cc:119-169, 372-406): the block at
+ * (r, c) of the nrb x ncb block grid holds tile order[r*ncb + c], each tile
+ * row-major rpb x cpb.  unpack=0: big[(r*rpb+rr)*ncb*cpb + c*cpb+cc] =
+ * tiles[order*bs + rr*cpb + cc]; unpack=1 the reverse. */
+int dtcf_pack_blocks_f64_host(double *tiles, int64_t ntiles, const int64_t *order, int64_t nrb,
+                              int64_t ncb, int64_t rpb, int64_t cpb, double *big, int unpack);
+
 /* Host-pointer permute of consecutive dense tiles (drop-in transpose). */
 int dtcf_permute_tiles_f64_host(const double *in, double *out, int dims, const int64_t *w,
                                 const int32_t *out_src, int64_t ntiles);
