@@ -1,0 +1,307 @@
+"""Block-sparse tensors under an abelian point group (D2h) and the grouped
+launch of a term's block pairs.
+
+Orbitals of each space (occupied ``o``, virtual ``v``) are split into 8
+irreps with seeded counts; irrep labels are 3-bit codes whose product is
+XOR, so a block of a tensor with irreps (g1, ..., gd) is symmetry-allowed iff
+g1 ^ ... ^ gd == 0 (about 1/8 of the blocks).  Only allowed blocks are
+stored: contiguously, in lexicographic order of their irrep tuples, each a
+dense row-major array over its index ranges.
+
+``BlockTerm`` turns ``C[c_idx] += alpha * A[a_idx] * B[b_idx]`` into the op
+and pair lists of ``dtcf_grouped_plan``: for every allowed C block, every
+assignment of irreps to the contracted indices that makes both the A block
+and the B block allowed gives one pair.  The GEMM axes are the A-external
+indices (rows, in C's order), the B-external indices (columns) and the
+contracted indices (in A's order); each block's strides are folded into the
+descriptors, so no block is ever transposed in memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+NIRREP = 8
+
+
+class Axis(C.Structure):
+    _fields_ = [("nd", C.c_int32), ("ext", C.c_int32 * 3), ("stride", C.c_int64 * 3)]
+
+
+class BlockOp(C.Structure):
+    _fields_ = [("base", C.c_int64), ("rows", Axis), ("cols", Axis)]
+
+
+class BlockPair(C.Structure):
+    _fields_ = [("c_op", C.c_int32), ("a_op", C.c_int32), ("b_op", C.c_int32)]
+
+
+def _declare():
+    L = lib()
+    vp = C.c_void_p
+    L.dtcf_grouped_plan.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int, C.POINTER(vp)]
+    L.dtcf_grouped_info.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64)]
+    L.dtcf_grouped_run.argtypes = [vp, C.c_double, vp, vp, vp, vp]
+    L.dtcf_grouped_destroy.argtypes = [vp]
+    for f in ("dtcf_grouped_plan", "dtcf_grouped_info", "dtcf_grouped_run", "dtcf_grouped_destroy"):
+        getattr(L, f).restype = C.c_int
+
+
+_declare()
+
+
+def irrep_counts(total: int, seed: int, lo: int, hi: int):
+    """Seeded per-irrep orbital counts in [lo, hi] summing to total."""
+    rng = np.random.default_rng(seed)
+    while True:
+        c = rng.integers(lo, hi + 1, size=NIRREP)
+        diff = total - int(c.sum())
+        i = 0
+        while diff != 0 and i < 10000:
+            j = rng.integers(NIRREP)
+            step = 1 if diff > 0 else -1
+            if lo <= c[j] + step <= hi:
+                c[j] += step
+                diff -= step
+            i += 1
+        if diff == 0:
+            return [int(x) for x in c]
+
+
+class Space:
+    def __init__(self, occ_counts, vir_counts):
+        self.counts = {"o": list(occ_counts), "v": list(vir_counts)}
+
+    @property
+    def o(self):
+        return sum(self.counts["o"])
+
+    @property
+    def v(self):
+        return sum(self.counts["v"])
+
+    @staticmethod
+    def seeded(o=64, v=256, seed=5):
+        return Space(irrep_counts(o, seed, max(1, o // 16), max(2, o // 5)),
+                     irrep_counts(v, seed + 1, max(1, v // 16), max(2, v // 5)))
+
+
+class BSTensor:
+    """Allowed blocks of a tensor whose dims have kinds in {'o','v'}."""
+
+    def __init__(self, space: Space, kinds: str, device="cuda", dtype=torch.float64):
+        self.space = space
+        self.kinds = kinds
+        self.blocks = []       # irrep tuples
+        self.offset = {}       # irreps -> element offset
+        self.shape = {}        # irreps -> extents
+        off = 0
+        for g in itertools.product(range(NIRREP), repeat=len(kinds)):
+            x = 0
+            for gi in g:
+                x ^= gi
+            if x != 0:
+                continue
+            shp = tuple(space.counts[k][gi] for k, gi in zip(kinds, g))
+            self.blocks.append(g)
+            self.offset[g] = off
+            self.shape[g] = shp
+            off += int(np.prod(shp))
+        self.size = off
+        self.data = torch.zeros(off, dtype=dtype, device=device) if device else None
+
+    def fill_normal(self, seed, sigma=0.05):
+        from .synth import normal
+        self.data.copy_(torch.from_numpy(normal(self.size, seed, sigma)))
+        return self
+
+    def block(self, g, host=None):
+        buf = self.data.cpu().numpy() if host is None else host
+        o = self.offset[g]
+        return buf[o:o + int(np.prod(self.shape[g]))].reshape(self.shape[g])
+
+
+def _axis(exts, strides):
+    a = Axis()
+    a.nd = len(exts)
+    for i, (e, s) in enumerate(zip(exts, strides)):
+        a.ext[i] = int(e)
+        a.stride[i] = int(s)
+    return a
+
+
+def _strides(shape):
+    st = [1] * len(shape)
+    for d in range(len(shape) - 2, -1, -1):
+        st[d] = st[d + 1] * shape[d + 1]
+    return st
+
+
+class BlockTerm:
+    """C[c_idx] += alpha * A[a_idx] * B[b_idx] over allowed blocks."""
+
+    def __init__(self, Ct: BSTensor, c_idx: str, At: BSTensor, a_idx: str, Bt: BSTensor,
+                 b_idx: str, cfg: int | None = None):
+        assert len(c_idx) == len(Ct.kinds) and len(a_idx) == len(At.kinds) and len(b_idx) == len(Bt.kinds)
+        self.C, self.A, self.B = Ct, At, Bt
+        self.c_idx, self.a_idx, self.b_idx = c_idx, a_idx, b_idx
+        contr = [x for x in a_idx if x in b_idx]
+        a_ext = [x for x in c_idx if x in a_idx]
+        b_ext = [x for x in c_idx if x in b_idx]
+        assert sorted(a_ext + b_ext) == sorted(c_idx), "C must hold the external indices"
+        assert 1 <= len(a_ext) <= 3 and 1 <= len(b_ext) <= 3 and 1 <= len(contr) <= 3
+        self.contr, self.a_ext, self.b_ext = contr, a_ext, b_ext
+        kind = {}
+        for idx, t in ((a_idx, At), (b_idx, Bt), (c_idx, Ct)):
+            for x, k in zip(idx, t.kinds):
+                assert kind.setdefault(x, k) == k, f"index {x} used with two kinds"
+        self.kind = kind
+        ops, pairs = [], []
+        opid = {}
+
+        def op_for(t: BSTensor, tag, idx, g, row_names, col_names):
+            key = (tag, g, tuple(row_names), tuple(col_names))
+            if key in opid:
+                return opid[key]
+            shp = t.shape[g]
+            st = _strides(shp)
+            pos = {x: i for i, x in enumerate(idx)}
+            op = BlockOp()
+            op.base = t.offset[g]
+            op.rows = _axis([shp[pos[x]] for x in row_names], [st[pos[x]] for x in row_names])
+            op.cols = _axis([shp[pos[x]] for x in col_names], [st[pos[x]] for x in col_names])
+            opid[key] = len(ops)
+            ops.append(op)
+            return opid[key]
+
+        self.flops = 0.0
+        for gc in Ct.blocks:
+            irr = dict(zip(c_idx, gc))
+            cop = op_for(Ct, "C", c_idx, gc, a_ext, b_ext)
+            for gk in itertools.product(range(NIRREP), repeat=len(contr)):
+                irr.update(zip(contr, gk))
+                ga = tuple(irr[x] for x in a_idx)
+                gb = tuple(irr[x] for x in b_idx)
+                if ga not in At.offset or gb not in Bt.offset:
+                    continue
+                aop = op_for(At, "A", a_idx, ga, a_ext, contr)
+                bop = op_for(Bt, "B", b_idx, gb, b_ext, contr)
+                p = BlockPair()
+                p.c_op, p.a_op, p.b_op = cop, aop, bop
+                pairs.append(p)
+        self.ops = (BlockOp * len(ops))(*ops)
+        self.pairs = (BlockPair * len(pairs))(*pairs)
+        self.npairs = len(pairs)
+        self.nops = len(ops)
+        self.cfg = pick_config(self) if cfg is None else cfg
+        self._plan()
+
+    def _plan(self):
+        h = C.c_void_p()
+        check(lib().dtcf_grouped_plan(C.cast(self.ops, C.c_void_p), self.nops,
+                                      C.cast(self.pairs, C.c_void_p), self.npairs, int(self.cfg),
+                                      C.byref(h)))
+        self.h = h
+        fl = C.c_double()
+        nw = C.c_int64()
+        spans = (C.c_int64 * 3)()
+        check(lib().dtcf_grouped_info(h, C.byref(fl), C.byref(nw), spans))
+        self.flops = fl.value
+        self.nwork = nw.value
+        assert spans[0] <= self.A.size and spans[1] <= self.B.size and spans[2] <= self.C.size
+
+    def replan(self, cfg):
+        lib().dtcf_grouped_destroy(self.h)
+        self.cfg = cfg
+        self._plan()
+
+    def run(self, alpha=1.0, A=None, B=None, Cb=None):
+        A = self.A.data if A is None else A
+        B = self.B.data if B is None else B
+        Cb = self.C.data if Cb is None else Cb
+        check(lib().dtcf_grouped_run(self.h, float(alpha), C.c_void_p(A.data_ptr()),
+                                     C.c_void_p(B.data_ptr()), C.c_void_p(Cb.data_ptr()),
+                                     C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().dtcf_grouped_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- autotuner
+_TUNE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_grouped.json")
+_TUNED = None
+
+
+def _bucket(term):
+    """Size bucket of a term: log2 of the mean C-block rows, cols and K."""
+    ops = term.ops
+    rows = cols = ks = 0.0
+    for p in term.pairs[:min(term.npairs, 4096)]:
+        c, a = ops[p.c_op], ops[p.a_op]
+        r = np.prod([c.rows.ext[i] for i in range(c.rows.nd)])
+        cc = np.prod([c.cols.ext[i] for i in range(c.cols.nd)])
+        k = np.prod([a.cols.ext[i] for i in range(a.cols.nd)])
+        rows += r
+        cols += cc
+        ks += k
+    n = max(1, min(term.npairs, 4096))
+    return f"{int(np.log2(max(rows / n, 1)))}-{int(np.log2(max(cols / n, 1)))}-{int(np.log2(max(ks / n, 1)))}"
+
+
+def pick_config(term):
+    global _TUNED
+    if _TUNED is None:
+        try:
+            with open(_TUNE_PATH) as f:
+                _TUNED = json.load(f)
+        except (OSError, ValueError):
+            _TUNED = {}
+    b = _bucket(term)
+    if b in _TUNED:
+        return int(_TUNED[b])
+    # default: large C blocks amortise better over 128x128 tiles
+    r, c, _ = (int(x) for x in b.split("-"))
+    return 1 if r >= 8 and c >= 8 else 0
+
+
+def autotune(term, reps=3):
+    """Time every tile configuration on the device; record the best for the
+    term's size bucket in tuned_grouped.json and return (cfg, times)."""
+    global _TUNED
+    times = {}
+    for cfg in (0, 1):
+        term.replan(cfg)
+        term.run(0.0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            term.run(0.0)
+        e1.record()
+        torch.cuda.synchronize()
+        times[cfg] = e0.elapsed_time(e1) / reps
+    best = min(times, key=times.get)
+    term.replan(best)
+    if _TUNED is None:
+        _TUNED = {}
+    _TUNED[_bucket(term)] = best
+    return best, times
+
+
+def save_tuning(path=None):
+    with open(path or _TUNE_PATH, "w") as f:
+        json.dump(_TUNED or {}, f, indent=1, sort_keys=True)

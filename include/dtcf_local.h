@@ -262,7 +262,49 @@ int dtcf_dltc_transpose_f64(const double *buf_x, const int32_t *x, double *buf_y
                             void *stream);
 
 /* ------------------------------------------------------------------ */
-/* 5. FP64 reductions and amplitude update (new; no reference          */
+/* 5. Grouped contraction over uneven block pairs (one launch per term)  */
+/*                                                                     */
+/* Replaces, for block-sparse tensors, the per-tile loop of            */
+/* transpose_and_dgemm (contraction.cc:1824-1956) and the tile-task loop */
+/* of DLTC (task.cc:97-133): every (A block, B block) pair of every C    */
+/* block runs inside one persistent launch, the block permutations folded */
+/* into the operand loads.  Each block is addressed through an affine    */
+/* descriptor: along an axis with digits d0 (most significant) .. d(nd-1) */
+/* of extents ext[], index x decomposes row-major and its element offset  */
+/* is sum d_i * stride[i].  A pair computes                              */
+/*   C_op[rows x cols] += alpha * A_op[rows x K] * B_op[cols x K]^T      */
+/* with A_op.rows = C_op.rows, B_op.rows = C_op.cols, A_op.cols = B_op.cols. */
+/* Pairs of one C op are summed in the order given (deterministic).      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int32_t nd;          /* 1..3 */
+    int32_t ext[3];
+    int64_t stride[3];
+} dtcf_axis;
+
+typedef struct {
+    int64_t base;        /* element offset of the block in its tensor's buffer */
+    dtcf_axis rows, cols;
+} dtcf_block_op;
+
+typedef struct {
+    int32_t c_op, a_op, b_op; /* indices into the op list */
+} dtcf_block_pair;
+
+typedef struct dtcf_grouped_s *dtcf_grouped;
+
+/* cfg: 0 = 64x64 C tiles per work item, 1 = 128x128 (autotuned per term). */
+int dtcf_grouped_plan(const dtcf_block_op *ops, int64_t nops, const dtcf_block_pair *pairs,
+                      int64_t npairs, int cfg, dtcf_grouped *out);
+/* Algorithmic flops (sum of 2*m*n*k over pairs), work items, and the buffer
+ * sizes (elements) the ops address in A, B and C. */
+int dtcf_grouped_info(dtcf_grouped g, double *flops, int64_t *nwork, int64_t *spans);
+int dtcf_grouped_run(dtcf_grouped g, double alpha, const double *A, const double *B, double *C,
+                     void *stream);
+int dtcf_grouped_destroy(dtcf_grouped g);
+
+/* ------------------------------------------------------------------ */
+/* 6. FP64 reductions and amplitude update (new; no reference          */
 /*    counterpart -- the reference stops at the contraction).          */
 /*    Layouts: R, T as [v, v, o, o]; V as [o, o, v, v].                 */
 /* ------------------------------------------------------------------ */

@@ -463,3 +463,36 @@ def dltc_task_sum(c_ids, Cd, a_ids, Ad, coef, wbO, wbV, nocc=12):
         box = _box(c_ids, ext, wbO, wbV, nocc)
         Cd[box] += coef * buf.reshape(Cd[box].shape)
     return Cd
+
+
+# --------------------------------------------------------------------------
+# block-sparse (abelian point group) contraction, block by block (numpy)
+# --------------------------------------------------------------------------
+def bs_contract(c_blocks, Cbuf, c_idx, a_blocks, Abuf, a_idx, b_blocks, Bbuf, b_idx, alpha,
+                nirrep=8):
+    """C[c_idx] += alpha * A[a_idx] * B[b_idx] over stored blocks.
+
+    *_blocks: dict irrep-tuple -> (offset, shape) of the stored (allowed)
+    blocks.  For every C block and every irrep assignment of the contracted
+    indices whose A and B blocks are stored, C_block += alpha *
+    einsum(A_block, B_block) -- the dense contraction restricted to the
+    symmetry-allowed blocks, summed pair by pair.  Pinned in tests by the
+    dense einsum of the zero-filled tensors."""
+    import itertools
+    contr = [x for x in a_idx if x in b_idx]
+    spec = f"{a_idx},{b_idx}->{c_idx}"
+    for gc, (co, cs) in c_blocks.items():
+        irr = dict(zip(c_idx, gc))
+        Cb = Cbuf[co:co + int(np.prod(cs))].reshape(cs)
+        for gk in itertools.product(range(nirrep), repeat=len(contr)):
+            irr.update(zip(contr, gk))
+            ga = tuple(irr[x] for x in a_idx)
+            gb = tuple(irr[x] for x in b_idx)
+            if ga not in a_blocks or gb not in b_blocks:
+                continue
+            ao, ash = a_blocks[ga]
+            bo, bsh = b_blocks[gb]
+            Ab = Abuf[ao:ao + int(np.prod(ash))].reshape(ash)
+            Bb = Bbuf[bo:bo + int(np.prod(bsh))].reshape(bsh)
+            Cb += alpha * np.einsum(spec, Ab, Bb, optimize=True)
+    return Cbuf
