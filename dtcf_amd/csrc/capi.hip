@@ -42,13 +42,7 @@ int pick_config(int64_t m, int64_t n, int64_t k)
 
 using namespace dtcf;
 
-struct dtcf_view_s {
-    int64_t rows, cols, rpb, cpb, ncb, nblk;
-    int64_t *rofs = nullptr, *cofs = nullptr, *blk = nullptr;  // device
-    int64_t span = 0;
-    bool colpairs = false, rowpairs = false;
-    bool owner = true;  // sub-views share their parent's tables
-};
+#include "view.hpp"
 
 extern "C" {
 

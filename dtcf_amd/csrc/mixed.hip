@@ -1,0 +1,525 @@
+// Mixed-precision contractions with FP64 recombination, and device-side
+// synthetic inputs.
+//
+// An FP64 operand (a dtcf_view, rows x K) is split once into low-precision
+// slices laid out [slice][rows_padded][K_padded] (K contiguous, padded with
+// zeros to multiples of 128 so the GEMM needs no bounds checks):
+//   F32   : one FP32 slice                              (FP32 MFMA)
+//   BF16x3: x = s1 + s2 + s3 + r, s_i = bf16(rest)      (BF16 MFMA, FP32 accum)
+//   F16x3 : the same in FP16 after scaling each row by a power of two so its
+//           largest magnitude is near 2^14 (keeps the slices normal)
+//   I8xS  : each row scaled by 2^-e (e = exponent of its max |x| + 1) and cut
+//           into S signed 7-bit digits, q_i = trunc(r_{i-1} * 2^(7i)),
+//           x = 2^e * sum_i q_i 2^(-7i) + O(2^(e-7S))   (INT8 MFMA, INT32 accum)
+// A product of slices (i, j) is one low-precision GEMM whose accumulators
+// are converted to FP64, scaled (2^(ea_r + eb_c - 7(i+j)) for I8, 2^(ea_r +
+// eb_c) for F16) and added to the FP64 C in the epilogue.  INT8 products are
+// exact in INT32 for K < 2^31 / 127^2 = 133,144, so the only I8 errors are
+// the dropped digits and the FP64 recombination (the Ozaki scheme of Ozaki,
+// Ogita, Oishi and Rump, "Error-free transformations of matrix
+// multiplication by using fast routines of matrix multiplication and its
+// applications", Numer. Algorithms 59 (2012), with integer slices).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "view.hpp"
+
+namespace dtcf {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------------ synth
+__device__ inline uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Same counter-based recipe as dtcf_amd/synth.py (splitmix64 -> Box-Muller).
+__global__ void fill_normal_kernel(double *out, int64_t n, uint64_t seed, double sigma,
+                                   int64_t start)
+{
+    const uint64_t base = seed << 40;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = (uint64_t)(start + i), pair = e >> 1;
+        const uint64_t h1 = splitmix64(base + (pair << 1));
+        const uint64_t h2 = splitmix64(base + (pair << 1) + 1);
+        const double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+        const double r = sqrt(-2.0 * log(u1)), t = 2.0 * M_PI * u2;
+        out[i] = sigma * ((e & 1) ? r * sin(t) : r * cos(t));
+    }
+}
+
+__global__ void fill_uniform_kernel(double *out, int64_t n, uint64_t seed, double lo, double hi)
+{
+    const uint64_t base = seed << 40;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t h = splitmix64(base + (uint64_t)i + (1ull << 39));
+        out[i] = lo + (hi - lo) * ((double)(h >> 11) * (1.0 / 9007199254740992.0));
+    }
+}
+
+// out = (x - x^{01} - x^{23} + x^{01,23}) / 2 for a [d0,d1,d2,d3] tensor.
+__global__ void antisym4_kernel(const double *__restrict__ x, double *__restrict__ out, int64_t d0,
+                                int64_t d1, int64_t d2, int64_t d3)
+{
+    const int64_t n = d0 * d1 * d2 * d3;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = e;
+        const int64_t l = t % d3;
+        t /= d3;
+        const int64_t k = t % d2;
+        t /= d2;
+        const int64_t j = t % d1;
+        const int64_t i = t / d1;
+        auto at = [&](int64_t a, int64_t b, int64_t c, int64_t d) {
+            return x[((a * d1 + b) * d2 + c) * d3 + d];
+        };
+        out[e] = 0.5 * (at(i, j, k, l) - at(j, i, k, l) - at(i, j, l, k) + at(j, i, l, k));
+    }
+}
+
+// ------------------------------------------------------------------ split
+struct ViewArgs {
+    const int64_t *rofs, *cofs, *blk;
+    FastDiv div_rpb, div_cpb;
+    int64_t ncb;
+};
+
+__device__ inline int64_t vaddr(const ViewArgs &v, int64_t r, int64_t c)
+{
+    int64_t a = v.rofs[r] + v.cofs[c];
+    if (v.blk) a += v.blk[(int64_t)v.div_rpb.div((uint32_t)r) * v.ncb + v.div_cpb.div((uint32_t)c)];
+    return a;
+}
+
+// One wave per row: max |x| (for the scaled modes), then the slices.
+template <int MODE>
+__global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ src, ViewArgs v,
+                                                   int64_t rows, int64_t K, int64_t rows_p,
+                                                   int64_t K_p, int nslice, void *__restrict__ dst,
+                                                   int32_t *__restrict__ rexp)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows_p) return;
+    const bool real = row < rows;
+    int e = 0;
+    if (MODE == DTCF_PREC_F16X3 || MODE == DTCF_PREC_I8) {
+        double m = 0.0;
+        for (int64_t c = lane; real && c < K; c += 64) m = fmax(m, fabs(src[vaddr(v, row, c)]));
+        for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        if (m > 0.0) {
+            int ex;
+            frexp(m, &ex);  // m = f * 2^ex, f in [0.5, 1): m < 2^ex
+            e = ex;
+        }
+        if (MODE == DTCF_PREC_F16X3) e -= 14;  // scaled rows have max |x| < 2^14
+        if (lane == 0) rexp[row] = e;
+    }
+    const size_t plane = (size_t)rows_p * K_p;
+    for (int64_t c = lane; c < K_p; c += 64) {
+        const double x = (real && c < K) ? src[vaddr(v, row, c)] : 0.0;
+        const size_t o = (size_t)row * K_p + c;
+        if constexpr (MODE == DTCF_PREC_F32) {
+            ((float *)dst)[o] = (float)x;
+        } else if constexpr (MODE == DTCF_PREC_BF16X3) {
+            __bf16 *d = (__bf16 *)dst;
+            double r = x;
+            for (int s = 0; s < 3; s++) {
+                const __bf16 q = (__bf16)(float)r;  // round to nearest (via fp32)
+                d[s * plane + o] = q;
+                r -= (double)(float)q;
+            }
+        } else if constexpr (MODE == DTCF_PREC_F16X3) {
+            _Float16 *d = (_Float16 *)dst;
+            double r = ldexp(x, -e);
+            for (int s = 0; s < 3; s++) {
+                const _Float16 q = (_Float16)(float)r;
+                d[s * plane + o] = q;
+                r -= (double)(float)q;
+            }
+        } else {  // I8
+            int8_t *d = (int8_t *)dst;
+            double r = ldexp(x, -e);  // |r| < 1
+            for (int s = 0; s < nslice; s++) {
+                const double t = trunc(ldexp(r, 7));  // |t| <= 127
+                d[s * plane + o] = (int8_t)t;
+                r = ldexp(r, 7) - t;                  // exact; |r| < 1 again
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ GEMM
+// Tiles 128 x 128 x (128 bytes of K), 256 threads, 4 waves of 64 x 64 as
+// 4 x 4 MFMA 16x16 accumulators.  Operand slices are [rows_p][K_p] with K
+// contiguous.  LDS is fragment-major: [row/16][k step][lane][VB bytes],
+// lane = row%16 + 16*kq, each wave fragment read one contiguous span.
+template <int PREC>
+struct LP;
+template <>
+struct LP<DTCF_PREC_F32> {
+    typedef float T;
+    static constexpr int VB = 4;    // bytes per lane per k step
+    static constexpr int KS = 4;    // k per MFMA
+    typedef f4 Acc;
+    typedef float Frag;
+    __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct LP<DTCF_PREC_BF16X3> {
+    typedef __bf16 T;
+    static constexpr int VB = 16;
+    static constexpr int KS = 32;
+    typedef f4 Acc;
+    typedef bf8 Frag;
+    __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct LP<DTCF_PREC_F16X3> {
+    typedef _Float16 T;
+    static constexpr int VB = 16;
+    static constexpr int KS = 32;
+    typedef f4 Acc;
+    typedef h8 Frag;
+    __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct LP<DTCF_PREC_I8> {
+    typedef int8_t T;
+    static constexpr int VB = 16;
+    static constexpr int KS = 64;
+    typedef i4 Acc;
+    typedef i4 Frag;
+    __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+};
+
+struct LPArgs {
+    const void *A, *B;       // slice planes
+    const int32_t *ea, *eb;  // row exponents (scaled modes) or null
+    int shift;               // extra power of two subtracted (7*(i+j) for I8)
+    double alpha;
+    int64_t M, N, Kp;        // logical rows / cols, padded K
+    int64_t Mp, Np;          // padded rows (plane strides)
+    double *C;               // FP64 output
+    ViewArgs cv;             // C view (when C_VIEW)
+    int64_t ldc;
+    uint32_t tiles_n;
+};
+
+constexpr int LBM = 128, LBN = 128, LBYTES = 128;  // K bytes per tile row
+
+template <int PREC, bool CVIEW>
+__global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
+{
+    typedef LP<PREC> P;
+    constexpr int VB = P::VB;
+    constexpr int NSTEP = LBYTES / (4 * VB);     // k steps per tile (4 kq lanes per step)
+    constexpr int FRAG = 64 * VB;                // bytes per (group, step) fragment
+    constexpr int OPB = (LBM / 16) * NSTEP * FRAG;  // bytes per operand tile = 16 KiB
+    __shared__ __attribute__((aligned(16))) unsigned char smem[2 * 2 * OPB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const uint32_t tm = blockIdx.x / g.tiles_n, tn = blockIdx.x - tm * g.tiles_n;
+    const int64_t m0 = (int64_t)tm * LBM, n0 = (int64_t)tn * LBN;
+    const int64_t kb = g.Kp * (int64_t)sizeof(typename P::T);  // bytes per slice row
+    const unsigned char *Ab = (const unsigned char *)g.A + m0 * kb;
+    const unsigned char *Bb = (const unsigned char *)g.B + n0 * kb;
+
+    // Global 16-byte chunks: pass i (0..3), wave w, lane l ->
+    // row = 16*g + l%16 with g = 2*w + i/2, byte 64*(i%2) + 16*(l/16) of the
+    // tile row.  Each chunk holds 16/VB fragments' worth of lanes.
+    uint4 ra[4], rb[4];
+    auto gload = [&](int64_t kt) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int grp = 2 * wave + (i >> 1);
+            const int64_t off = (int64_t)(16 * grp + (lane & 15)) * kb + kt * LBYTES +
+                                64 * (i & 1) + 16 * (lane >> 4);
+            ra[i] = *(const uint4 *)(Ab + off);
+            rb[i] = *(const uint4 *)(Bb + off);
+        }
+    };
+    auto sstore = [&](unsigned char *sa, unsigned char *sb) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int grp = 2 * wave + (i >> 1);
+            const int byte = 64 * (i & 1) + 16 * (lane >> 4);  // byte in the 128-byte row
+            const int r = lane & 15;
+            if constexpr (VB == 16) {
+                // one chunk = one lane's fragment: step = byte/64, kq = (byte%64)/16
+                const int step = byte >> 6, kq = (byte & 63) >> 4;
+                const int o = ((grp * NSTEP + step) * 64 + r + 16 * kq) * VB;
+                *(uint4 *)(sa + o) = ra[i];
+                *(uint4 *)(sb + o) = rb[i];
+            } else {
+                // VB == 4: the chunk holds 4 consecutive k = one k step's 4 kq lanes
+                const int step = byte >> 4;
+                const uint32_t *xa = (const uint32_t *)&ra[i], *xb = (const uint32_t *)&rb[i];
+#pragma unroll
+                for (int kq = 0; kq < 4; kq++) {
+                    const int o = ((grp * NSTEP + step) * 64 + r + 16 * kq) * VB;
+                    *(uint32_t *)(sa + o) = xa[kq];
+                    *(uint32_t *)(sb + o) = xb[kq];
+                }
+            }
+        }
+    };
+
+    typename P::Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (typename P::Acc){0, 0, 0, 0};
+
+    const int64_t nk = g.Kp * (int64_t)sizeof(typename P::T) / LBYTES;
+    gload(0);
+    sstore(smem, smem + OPB);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; kt++) {
+        const int cur = (int)(kt & 1);
+        const bool more = kt + 1 < nk;
+        if (more) gload(kt + 1);
+        const unsigned char *sa = smem + cur * 2 * OPB, *sb = sa + OPB;
+#pragma unroll
+        for (int s = 0; s < NSTEP; s++) {
+            typename P::Frag af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                af[i] = *(const typename P::Frag *)(sa + (((wm * 4 + i) * NSTEP + s) * 64 + lane) * VB);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                bf[j] = *(const typename P::Frag *)(sb + (((wn * 4 + j) * NSTEP + s) * 64 + lane) * VB);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = P::mfma(af[i], bf[j], acc[i][j]);
+            if (s == NSTEP / 2 - 1 && more) {
+                unsigned char *na = smem + (cur ^ 1) * 2 * OPB;
+                sstore(na, na + OPB);
+            }
+        }
+        if (NSTEP == 1 && more) {
+            unsigned char *na = smem + (cur ^ 1) * 2 * OPB;
+            sstore(na, na + OPB);
+        }
+        __syncthreads();
+    }
+    // Epilogue: D[4*(l/16) + r][l%16] per 16x16 accumulator.
+    const int fr = lane & 15, fq = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t col = n0 + wn * 64 + j * 16 + fr;
+            if (col >= g.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wm * 64 + i * 16 + 4 * fq + r;
+                if (row >= g.M) continue;
+                double v = (double)acc[i][j][r];
+                if (g.ea) v = ldexp(v, g.ea[row] + g.eb[col] - g.shift);
+                int64_t addr;
+                if constexpr (CVIEW) addr = vaddr(g.cv, row, col);
+                else addr = row * g.ldc + col;
+                g.C[addr] += g.alpha * v;
+            }
+        }
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+}  // namespace dtcf
+
+using namespace dtcf;
+
+static int view_args(dtcf_view v, ViewArgs &a, int64_t &rows, int64_t &cols)
+{
+    DTCF_REQUIRE(v, "null view");
+    a.rofs = v->rofs;
+    a.cofs = v->cofs;
+    a.blk = v->blk;
+    a.div_rpb = FastDiv((uint32_t)v->rpb);
+    a.div_cpb = FastDiv((uint32_t)v->cpb);
+    a.ncb = v->ncb;
+    rows = v->rows;
+    cols = v->cols;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_fill_normal_f64(double *out, int64_t n, uint64_t seed, double sigma,
+                                    int64_t start, void *stream)
+{
+    DTCF_REQUIRE(n >= 0 && (n == 0 || out), "fill: bad buffer");
+    if (n == 0) return DTCF_OK;
+    hipLaunchKernelGGL(fill_normal_kernel, dim3(2048), dim3(256), 0, as_stream(stream), out, n,
+                       seed, sigma, start);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_fill_uniform_f64(double *out, int64_t n, uint64_t seed, double lo, double hi,
+                                     void *stream)
+{
+    DTCF_REQUIRE(n >= 0 && (n == 0 || out), "fill: bad buffer");
+    if (n == 0) return DTCF_OK;
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(256), dim3(256), 0, as_stream(stream), out, n,
+                       seed, lo, hi);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_antisymmetrize4_f64(const double *in, double *out, int64_t d0, int64_t d1,
+                                        int64_t d2, int64_t d3, void *stream)
+{
+    DTCF_REQUIRE(in && out && in != out, "antisym: needs distinct buffers");
+    DTCF_REQUIRE(d0 == d1 && d2 == d3 && d0 > 0 && d2 > 0, "antisym: pairs must have equal extents");
+    hipLaunchKernelGGL(antisym4_kernel, dim3(4096), dim3(256), 0, as_stream(stream), in, out, d0,
+                       d1, d2, d3);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+extern "C" int64_t dtcf_split_bytes(int prec, int nslice, int64_t rows, int64_t K)
+{
+    const int64_t rp = round_up(std::max<int64_t>(rows, 1), 128), kp = round_up(std::max<int64_t>(K, 1), 128);
+    switch (prec) {
+    case DTCF_PREC_F32: return rp * kp * 4;
+    case DTCF_PREC_BF16X3:
+    case DTCF_PREC_F16X3: return 3 * rp * kp * 2;
+    case DTCF_PREC_I8: return (int64_t)nslice * rp * kp;
+    default: return -1;
+    }
+}
+
+extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view v, void *dst,
+                              int32_t *rexp, void *stream)
+{
+    DTCF_REQUIRE(src && v && dst, "split: null argument");
+    DTCF_REQUIRE(prec >= DTCF_PREC_F32 && prec <= DTCF_PREC_I8, "split: unknown precision");
+    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(nslice >= 1 && nslice <= 7, "split: 1..7 INT8 slices");
+    if (prec == DTCF_PREC_F16X3 || prec == DTCF_PREC_I8) DTCF_REQUIRE(rexp, "split: needs rexp");
+    ViewArgs a{};
+    int64_t rows, K;
+    int st = view_args(v, a, rows, K);
+    if (st) return st;
+    const int64_t rp = round_up(rows, 128), kp = round_up(K, 128);
+    const unsigned grid = (unsigned)((rp + 3) / 4);
+    hipStream_t s = as_stream(stream);
+    switch (prec) {
+    case DTCF_PREC_F32:
+        hipLaunchKernelGGL(split_kernel<DTCF_PREC_F32>, dim3(grid), dim3(256), 0, s, src, a, rows, K,
+                           rp, kp, 1, dst, rexp);
+        break;
+    case DTCF_PREC_BF16X3:
+        hipLaunchKernelGGL(split_kernel<DTCF_PREC_BF16X3>, dim3(grid), dim3(256), 0, s, src, a, rows,
+                           K, rp, kp, 3, dst, rexp);
+        break;
+    case DTCF_PREC_F16X3:
+        hipLaunchKernelGGL(split_kernel<DTCF_PREC_F16X3>, dim3(grid), dim3(256), 0, s, src, a, rows,
+                           K, rp, kp, 3, dst, rexp);
+        break;
+    default:
+        hipLaunchKernelGGL(split_kernel<DTCF_PREC_I8>, dim3(grid), dim3(256), 0, s, src, a, rows, K,
+                           rp, kp, nslice, dst, rexp);
+        break;
+    }
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+template <int PREC, bool CV>
+static void lp_launch(const LPArgs &g, hipStream_t s)
+{
+    const unsigned grid = (unsigned)(((g.M + LBM - 1) / LBM) * g.tiles_n);
+    hipLaunchKernelGGL((lp_gemm_kernel<PREC, CV>), dim3(grid), dim3(256), 0, s, g);
+}
+
+extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k,
+                                   double alpha, const void *A_slices, const int32_t *ea,
+                                   const void *B_slices, const int32_t *eb, int max_level,
+                                   double *C, dtcf_view cv, int64_t ldc, void *stream)
+{
+    DTCF_REQUIRE(m >= 0 && n >= 0 && k >= 0, "gemm_split: negative size");
+    if (m == 0 || n == 0 || k == 0 || alpha == 0.0) return DTCF_OK;
+    DTCF_REQUIRE(A_slices && B_slices && C, "gemm_split: null operand");
+    DTCF_REQUIRE(prec >= DTCF_PREC_F32 && prec <= DTCF_PREC_I8, "gemm_split: unknown precision");
+    const bool scaled = prec == DTCF_PREC_F16X3 || prec == DTCF_PREC_I8;
+    if (scaled) DTCF_REQUIRE(ea && eb, "gemm_split: scaled slices need row exponents");
+    const int ns = prec == DTCF_PREC_F32 ? 1 : prec == DTCF_PREC_I8 ? nslice : 3;
+    DTCF_REQUIRE(ns >= 1 && ns <= 7, "gemm_split: bad slice count");
+    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(k < 133144, "gemm_split: INT8 products exact only for K < 133144");
+    LPArgs g{};
+    g.M = m;
+    g.N = n;
+    g.Mp = round_up(m, 128);
+    g.Np = round_up(n, 128);
+    g.Kp = round_up(k, 128);
+    g.alpha = alpha;
+    g.C = C;
+    g.ldc = ldc;
+    g.tiles_n = (uint32_t)(g.Np / LBN);
+    if (cv) {
+        int64_t r, c;
+        int st = view_args(cv, g.cv, r, c);
+        if (st) return st;
+        DTCF_REQUIRE(r == m && c == n, "gemm_split: C view is not m x n");
+    } else {
+        DTCF_REQUIRE(ldc >= n, "gemm_split: ldc < n");
+    }
+    const size_t esz = prec == DTCF_PREC_F32 ? 4 : prec == DTCF_PREC_I8 ? 1 : 2;
+    const size_t pa = (size_t)g.Mp * g.Kp * esz, pb = (size_t)g.Np * g.Kp * esz;
+    // Slice pairs (i, j), 1-based, with i + j <= max_level (default ns + 1).
+    const int lev = max_level > 0 ? max_level : ns + 1;
+    hipStream_t s = as_stream(stream);
+    for (int i = 1; i <= ns; i++)
+        for (int j = 1; j <= ns && i + j <= lev; j++) {
+            g.A = (const unsigned char *)A_slices + (i - 1) * pa;
+            g.B = (const unsigned char *)B_slices + (j - 1) * pb;
+            g.ea = scaled ? ea : nullptr;
+            g.eb = scaled ? eb : nullptr;
+            g.shift = prec == DTCF_PREC_I8 ? 7 * (i + j) : 0;
+            switch (prec) {
+            case DTCF_PREC_F32:
+                cv ? lp_launch<DTCF_PREC_F32, true>(g, s) : lp_launch<DTCF_PREC_F32, false>(g, s);
+                break;
+            case DTCF_PREC_BF16X3:
+                cv ? lp_launch<DTCF_PREC_BF16X3, true>(g, s) : lp_launch<DTCF_PREC_BF16X3, false>(g, s);
+                break;
+            case DTCF_PREC_F16X3:
+                cv ? lp_launch<DTCF_PREC_F16X3, true>(g, s) : lp_launch<DTCF_PREC_F16X3, false>(g, s);
+                break;
+            default:
+                cv ? lp_launch<DTCF_PREC_I8, true>(g, s) : lp_launch<DTCF_PREC_I8, false>(g, s);
+                break;
+            }
+            DTCF_HIP(hipGetLastError());
+        }
+    return DTCF_OK;
+}

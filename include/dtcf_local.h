@@ -304,7 +304,45 @@ int dtcf_grouped_run(dtcf_grouped g, double alpha, const double *A, const double
 int dtcf_grouped_destroy(dtcf_grouped g);
 
 /* ------------------------------------------------------------------ */
-/* 6. FP64 reductions and amplitude update (new; no reference          */
+/* 6. Mixed precision with FP64 recombination (new; the reference is    */
+/*    FP64 only) and device-side synthetic inputs                       */
+/* ------------------------------------------------------------------ */
+enum {
+    DTCF_PREC_F32 = 1,    /* one FP32 slice, FP32 MFMA, FP32 accumulation */
+    DTCF_PREC_BF16X3 = 2, /* three BF16 slices, BF16 MFMA, FP32 accumulation */
+    DTCF_PREC_F16X3 = 3,  /* three FP16 slices of row-scaled values, FP32 accumulation */
+    DTCF_PREC_I8 = 4      /* nslice signed 7-bit INT8 digits of row-scaled values,
+                             INT8 MFMA, exact INT32 accumulation (Ozaki scheme) */
+};
+/* Device bytes needed for the slices of a rows x K operand. */
+int64_t dtcf_split_bytes(int prec, int nslice, int64_t rows, int64_t K);
+/* Split an FP64 operand given as a view (rows x K) into slices laid out
+ * [slice][round_up(rows,128)][round_up(K,128)] (K contiguous, zero padded);
+ * rexp[round_up(rows,128)] receives the per-row power-of-two exponents for
+ * F16X3 and I8. */
+int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view v, void *dst, int32_t *rexp,
+                   void *stream);
+/* C (FP64) += alpha * sum over slice pairs (i, j), i + j <= max_level
+ * (<= 0: nslice + 1), of A_i * B_j^T, each pair one low-precision GEMM whose
+ * accumulators are converted to FP64 and rescaled in the epilogue.  A_slices
+ * from dtcf_split_f64 over an m x k view, B_slices over an n x k view.  C is
+ * a view (cv) or dense row-major with leading dimension ldc. */
+int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k, double alpha,
+                        const void *A_slices, const int32_t *ea, const void *B_slices,
+                        const int32_t *eb, int max_level, double *C, dtcf_view cv, int64_t ldc,
+                        void *stream);
+/* Counter-based N(0, sigma^2) (splitmix64 + Box-Muller; element index
+ * start + i), uniform [lo, hi), and (X - X^{01} - X^{23} + X^{01,23}) / 2 of a
+ * [d0,d1,d2,d3] tensor (d0 = d1, d2 = d3). */
+int dtcf_fill_normal_f64(double *out, int64_t n, uint64_t seed, double sigma, int64_t start,
+                         void *stream);
+int dtcf_fill_uniform_f64(double *out, int64_t n, uint64_t seed, double lo, double hi,
+                          void *stream);
+int dtcf_antisymmetrize4_f64(const double *in, double *out, int64_t d0, int64_t d1, int64_t d2,
+                             int64_t d3, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 7. FP64 reductions and amplitude update (new; no reference          */
 /*    counterpart -- the reference stops at the contraction).          */
 /*    Layouts: R, T as [v, v, o, o]; V as [o, o, v, v].                 */
 /* ------------------------------------------------------------------ */
