@@ -1,0 +1,169 @@
+"""Precision study: one CCD-like amplitude iteration in several precisions.
+
+System (o occupied, v virtual spin orbitals): occupied eps ~ U[-2, -0.5],
+virtual eps ~ U[0.2, 3]; integrals V_vvvv[a,b,c,d], V_vvoo[a,b,i,j] and
+W[k,j,c,b] antisymmetrised N(0, 0.05^2); MP2 amplitudes
+T[a,b,i,j] = V_vvoo / D.  One iteration:
+    R[a,b,i,j] = V_vvoo + 1/2 sum_cd V[a,b,c,d] T[c,d,i,j]              (ladder)
+                        + sum_kc T'[i,k,a,c] W[k,j,c,b]  -> R[a,b,i,j]  (ring)
+    T_new = R / D,   E = 1/4 sum V_oovv[i,j,a,b] T_new[a,b,i,j]
+where T' is T viewed as [i,k,a,c].  The two contractions run in the
+precision under study; the denominator update and the energy always run in
+FP64 on the device.  Errors are stated against the FP64 MFMA run of the same
+inputs (max |dR|, max |dR| / max |R|, |dE|).
+
+Run: python -m dtcf_amd.precision --o 64 --v 256 --out study.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import mixed, ops
+from ._lib import check, lib
+
+VARIANTS = [
+    ("fp64", None, 0, 0),
+    ("fp32", mixed.F32, 1, 0),
+    ("bf16x3", mixed.BF16X3, 3, 0),
+    ("fp16x3", mixed.F16X3, 3, 0),
+    ("int8x4", mixed.I8, 4, 0),
+    ("int8x5/L5", mixed.I8, 5, 5),
+    ("int8x5", mixed.I8, 5, 0),
+    ("int8x6", mixed.I8, 6, 0),
+]
+
+
+def _sync():
+    torch.cuda.synchronize()
+
+
+class System:
+    def __init__(self, o, v, dev="cuda", seed=100):
+        self.o, self.v = o, v
+        self.eo = mixed.fill_uniform(o, seed, -2.0, -0.5, dev)
+        self.ev = mixed.fill_uniform(v, seed + 1, 0.2, 3.0, dev)
+        raw = mixed.fill_normal(v ** 4, seed + 2, 0.05, dev)
+        self.Vvvvv = mixed.antisymmetrize4(raw, (v, v, v, v))
+        del raw
+        raw = mixed.fill_normal(v * v * o * o, seed + 3, 0.05, dev)
+        self.Vvvoo = mixed.antisymmetrize4(raw, (v, v, o, o))
+        raw = mixed.fill_normal(o * o * v * v, seed + 4, 0.05, dev)
+        self.W = mixed.antisymmetrize4(raw, (o, o, v, v))  # [k,j,c,b]
+        del raw
+        # V_oovv[i,j,a,b] = V_vvoo[a,b,i,j]
+        self.Voovv = ops.rrr_transpose(self.Vvvoo, [2, 3, 0, 1], [v, v, o, o])
+        self.T = ops.denom_update(self.Vvvoo, self.eo, self.ev)       # [c,d,i,j]
+        oo, vv = o * o, v * v
+        # ladder GEMM: C (ab x ij) += A (ab x cd) * B (ij x cd)^T
+        self.lad_a = ops.View(vv, vv, np.arange(vv) * vv, np.arange(vv))
+        self.lad_b = ops.View(oo, vv, np.arange(oo), np.arange(vv) * oo)
+        # ring GEMM over T'[i,k,a,c] (T stored [a,c,i,k] read as [i,k,a,c])
+        # rows (i,a), cols (c,k) with k fastest, so T's rows are K-contiguous;
+        # W rows (j,b) with b contiguous, cols (c,k); C rows (i,a), cols (j,b)
+        #   T[a,c,i,k] at ((a*v + c)*o + i)*o + k
+        ia = np.array([i * o + a * v * o * o for i in range(o) for a in range(v)])
+        kc = np.array([k + c * o * o for c in range(v) for k in range(o)])
+        jb = np.array([j * v * v + b for j in range(o) for b in range(v)])
+        kcw = np.array([k * o * v * v + c * v for c in range(v) for k in range(o)])
+        #   R[a,b,i,j] at ((a*v + b)*o + i)*o + j
+        ia_r = np.array([a * v * o * o + i * o for i in range(o) for a in range(v)])
+        jb_r = np.array([b * o * o + j for j in range(o) for b in range(v)])
+        n = o * v
+        self.ring_a = ops.View(n, n, ia, kc)
+        self.ring_b = ops.View(n, n, jb, kcw)
+        self.ring_c = ops.View(n, n, ia_r, jb_r)
+        self.lad_flops = 2.0 * vv * oo * vv
+        self.ring_flops = 2.0 * n * n * n
+        _sync()
+
+    def iterate(self, prec, nslice, level):
+        """Returns (R, T_new, E, seconds per phase).  Phases are timed with a
+        device synchronisation on each side, so they add up to the
+        contraction time."""
+        o, v = self.o, self.v
+        R = self.Vvvoo.clone()
+        ph = {}
+
+        def timed(name, fn):
+            _sync()
+            t = time.perf_counter()
+            r = fn()
+            _sync()
+            ph[name] = ph.get(name, 0.0) + time.perf_counter() - t
+            return r
+
+        if prec is None:
+            timed("gemm_ladder", lambda: ops.gemm_view(v * v, o * o, v * v, 0.5, self.Vvvvv,
+                                                       self.lad_a, self.T, self.lad_b, R, None,
+                                                       ldc=o * o))
+            timed("gemm_ring", lambda: ops.gemm_view(o * v, o * v, o * v, 1.0, self.T, self.ring_a,
+                                                     self.W, self.ring_b, R, self.ring_c))
+        else:
+            sa = timed("split_ladder", lambda: mixed.Split(prec, self.Vvvvv, self.lad_a, nslice))
+            sb = timed("split_ladder", lambda: mixed.Split(prec, self.T, self.lad_b, nslice))
+            timed("gemm_ladder", lambda: mixed.gemm_split(sa, sb, R, alpha=0.5, ldc=o * o,
+                                                          max_level=level))
+            del sa, sb
+            ra = timed("split_ring", lambda: mixed.Split(prec, self.T, self.ring_a, nslice))
+            rb = timed("split_ring", lambda: mixed.Split(prec, self.W, self.ring_b, nslice))
+            timed("gemm_ring", lambda: mixed.gemm_split(ra, rb, R, alpha=1.0, cview=self.ring_c,
+                                                        max_level=level))
+            del ra, rb
+        Tn = ops.denom_update(R, self.eo, self.ev)
+        E = ops.energy(self.Voovv, Tn, o, v).item()
+        return R, Tn, E, ph
+
+
+def study(o, v, variants=VARIANTS, reps=1, log=print):
+    sysm = System(o, v)
+    flops = sysm.lad_flops + sysm.ring_flops
+    out = {"o": o, "v": v, "flops_ladder": sysm.lad_flops, "flops_ring": sysm.ring_flops,
+           "variants": []}
+    R64 = E64 = t64 = None
+    for name, prec, ns, lev in variants:
+        best = None
+        for _ in range(max(1, reps)):
+            R, Tn, E, ph = sysm.iterate(prec, ns, lev)
+            tc = sum(ph.values())
+            if best is None or tc < best[0]:
+                best = (tc, ph)
+        tc, ph = best
+        rec = dict(name=name, seconds=tc, phases={k: round(x, 5) for k, x in ph.items()},
+                   tflops=flops / tc / 1e12, energy=E)
+        if prec is None:
+            R64, E64, t64 = R, E, tc
+            rec.update(max_abs_err=0.0, max_rel_err=0.0, energy_err=0.0, speedup=1.0, products=1)
+        else:
+            d = (R - R64).abs().max().item()
+            rec.update(max_abs_err=d, max_rel_err=d / R64.abs().max().item(),
+                       energy_err=abs(E - E64), speedup=t64 / tc,
+                       products=mixed.n_products(prec, ns, lev))
+        out["variants"].append(rec)
+        log(json.dumps(rec))
+        del R, Tn
+        torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--o", type=int, default=64)
+    ap.add_argument("--v", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    check(lib().dtcf_init(0))
+    res = study(args.o, args.v, reps=args.reps, log=lambda s: print(s, flush=True))
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
