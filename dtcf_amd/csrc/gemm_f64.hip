@@ -523,6 +523,18 @@ int launch_cfg(GemmArgs g, hipStream_t s)
 
 }  // namespace
 
+int gemm_reserve_workspace(hipStream_t s)
+{
+    // rem * sk <= slots for every configuration (sk <= slots / rem), so the
+    // largest tile area times the largest slot count bounds every request.
+    int cus = 0;
+    int st = device_cus(&cus);
+    if (st != DTCF_OK) return st;
+    const size_t tile = std::max<size_t>({128 * 128, 192 * 96, 96 * 192, 64 * 64});
+    double *ws = nullptr;
+    return split_workspace(s, (size_t)cus * 2 * tile * sizeof(double), &ws);
+}
+
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s)
 {
     if (g.m <= 0 || g.n <= 0 || g.k <= 0) return DTCF_OK;

@@ -47,4 +47,8 @@ struct GemmArgs {
 
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
 
+// Allocates the tail-split workspace of stream `s` at the largest size any
+// GEMM on it can need, so later launches on `s` never allocate.
+int gemm_reserve_workspace(hipStream_t s);
+
 }  // namespace dtcf

@@ -174,7 +174,6 @@ template <>
 struct LP<DTCF_PREC_F32> {
     typedef float T;
     static constexpr int VB = 4;    // bytes per lane per k step
-    static constexpr int KS = 4;    // k per MFMA
     typedef f4 Acc;
     typedef float Frag;
     __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
@@ -186,7 +185,6 @@ template <>
 struct LP<DTCF_PREC_BF16X3> {
     typedef __bf16 T;
     static constexpr int VB = 16;
-    static constexpr int KS = 32;
     typedef f4 Acc;
     typedef bf8 Frag;
     __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
@@ -198,7 +196,6 @@ template <>
 struct LP<DTCF_PREC_F16X3> {
     typedef _Float16 T;
     static constexpr int VB = 16;
-    static constexpr int KS = 32;
     typedef f4 Acc;
     typedef h8 Frag;
     __device__ static inline Acc mfma(Frag a, Frag b, Acc c)
@@ -210,7 +207,6 @@ template <>
 struct LP<DTCF_PREC_I8> {
     typedef int8_t T;
     static constexpr int VB = 16;
-    static constexpr int KS = 64;
     typedef i4 Acc;
     typedef i4 Frag;
     __device__ static inline Acc mfma(Frag a, Frag b, Acc c)

@@ -355,6 +355,82 @@ int dtcf_energy_f64(const double *V, const double *T, int64_t o, int64_t v, doub
 /* *out = sqrt(sum R^2)   (deterministic order) */
 int dtcf_norm2_f64(const double *R, int64_t n, double *out_dev, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* 8. Serving: device-resident CC iteration programs under an HBM      */
+/*    budget, continuous batching over streams.  Replaces DLTC's task  */
+/*    pool and tile cache (DLTC/task.cc:24-137 task_mul/task_sum and   */
+/*    the Scioto pool, DLTC/cache.h:23-133 LRU tile cache,             */
+/*    DLTC/exec.cc:58-110 task granularity) for one GPU.               */
+/* ------------------------------------------------------------------ */
+typedef struct dtcf_server_s *dtcf_server;
+typedef struct dtcf_program_s *dtcf_program;
+
+/* Opens a server on the current device: one device arena of `budget`
+ * bytes (resident tensors and request blocks are carved out of it) and
+ * `nslots` (1..16) non-blocking streams of default priority.  Each slot
+ * also keeps a GEMM tail-split workspace (under 80 MB) outside the arena. */
+int dtcf_server_open(int64_t budget, int nslots, dtcf_server *out);
+/* Waits for every admitted request, then frees everything. */
+int dtcf_server_close(dtcf_server s);
+/* Resident tensor storage from the arena (zeroed).  Freed by
+ * dtcf_server_free or at close. */
+int dtcf_server_alloc(dtcf_server s, int64_t bytes, void **dev_ptr);
+int dtcf_server_free(dtcf_server s, void *dev_ptr);
+/* bytes: arena size, in use by resident tensors, reserved by admitted
+ * requests; peak_reserved: high-water mark of the request reservations. */
+int dtcf_server_usage(dtcf_server s, int64_t *budget, int64_t *resident, int64_t *reserved,
+                      int64_t *peak_reserved);
+
+/* A program is the statement list of one term file over one system's
+ * extents (a CC residual evaluation).  Arrays are dense row-major FP64.
+ * Index lists are comma-separated index names, e.g. "p1a,h2a".
+ * prec: 0 = FP64 MFMA, else a DTCF_PREC_* mode for every contraction
+ * (nslice, max_level as in dtcf_gemm_split_f64); sums stay FP64. */
+enum { DTCF_ARR_INPUT = 0, DTCF_ARR_TEMP = 1, DTCF_ARR_OUTPUT = 2 };
+int dtcf_program_create(dtcf_server s, int prec, int nslice, int max_level, dtcf_program *out);
+int dtcf_program_array(dtcf_program p, const char *name, int ndim, const int64_t *dims, int role);
+/* Inputs are resident tensors the caller owns (e.g. from dtcf_server_alloc). */
+int dtcf_program_bind(dtcf_program p, const char *name, const double *dev_ptr);
+/* C[c_idx] = alpha * sum_{contracted} A[a_idx] * B[b_idx] */
+int dtcf_program_contract(dtcf_program p, const char *c, const char *c_idx, const char *a,
+                          const char *a_idx, const char *b, const char *b_idx, double alpha);
+/* C[c_idx] = sum_t coef[t] * X_t[idx_t]  (each idx_t a permutation of c_idx) */
+int dtcf_program_sum(dtcf_program p, const char *c, const char *c_idx, int nterms,
+                     const char *const *names, const char *const *idx, const double *coefs);
+/* Plans every statement (operand views, sum strides), the dependency graph
+ * and the static placement of temporaries inside one request block. */
+int dtcf_program_finalize(dtcf_program p);
+/* block_bytes: arena bytes one request reserves; flops: 2*m*n*k summed over
+ * contractions plus one per summed element. */
+int dtcf_program_info(dtcf_program p, int64_t *block_bytes, int64_t *nstatements, double *flops,
+                      int64_t *noutputs);
+int dtcf_program_destroy(dtcf_program p);
+
+/* Requests: one evaluation of a finalized program.  `outputs` holds one
+ * device buffer per DTCF_ARR_OUTPUT array, in declaration order; they must
+ * stay valid until the request completes.  Requests are admitted in
+ * arrival order as their block fits in the arena (later, smaller requests
+ * may pass a blocked one, at most 8 times); the statements of all admitted
+ * requests are issued to free slots as their inputs complete, oldest
+ * request first.  A request's results do not depend on what runs beside it. */
+int dtcf_request_submit(dtcf_server s, dtcf_program p, double *const *outputs, int64_t *id);
+/* DTCF_OK when complete, DTCF_ERR_NOTREADY on timeout, another status if
+ * the request failed.  timeout_ms < 0 waits forever. */
+int dtcf_request_wait(dtcf_server s, int64_t id, double timeout_ms);
+typedef struct {
+    int32_t state;                      /* 0 queued, 1 admitted, 2 done, 3 failed */
+    int64_t ntasks;
+    double t_submit, t_admit, t_done;   /* microseconds since dtcf_server_open */
+    double flops;
+} dtcf_request_info;
+int dtcf_request_info_get(dtcf_server s, int64_t id, dtcf_request_info *info);
+/* Per statement: when its inputs were complete (ready), when it was issued
+ * to a slot and when its completion was observed (microseconds). */
+int dtcf_request_task_times(dtcf_server s, int64_t id, double *t_ready, double *t_issue,
+                            double *t_done, int64_t n);
+/* Forgets a completed request's record. */
+int dtcf_request_release(dtcf_server s, int64_t id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -496,3 +496,44 @@ def bs_contract(c_blocks, Cbuf, c_idx, a_blocks, Abuf, a_idx, b_blocks, Bbuf, b_
             Bb = Bbuf[bo:bo + int(np.prod(bsh))].reshape(bsh)
             Cb += alpha * np.einsum(spec, Ab, Bb, optimize=True)
     return Cbuf
+
+
+# --------------------------------------------------------------------------
+# term-list statements (the UHF CCSD lists), evaluated with numpy einsum
+# --------------------------------------------------------------------------
+def _letters(names):
+    table = {}
+    for n in names:
+        if n not in table:
+            table[n] = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"[len(table)]
+    return table
+
+
+def termlist_statement(kind, c_idx, terms, arrays):
+    """One statement of a term list.  kind "mul": terms = [(name, idx),
+    (name, idx)], result C[c_idx] = sum over the shared indices of A * B (the
+    DLTC multiplication, DLTC/expr.cc:13-67 with DLTC/compute.cc:3-36, as a
+    plain Einstein sum).  kind "sum": terms = [(name, idx, coef), ...],
+    result = sum of coef * X permuted to c_idx (DLTC/task.cc:51-81).  Pinned
+    by the brute-force loops of tests/test_termlist.py."""
+    if kind == "mul":
+        (an, ai), (bn, bi) = terms
+        L = _letters(list(c_idx) + list(ai) + list(bi))
+        spec = "".join(L[x] for x in ai) + "," + "".join(L[x] for x in bi) + "->" + \
+            "".join(L[x] for x in c_idx)
+        return np.einsum(spec, arrays[an], arrays[bn], optimize=False)
+    out = None
+    for name, idx, coef in terms:
+        perm = [list(idx).index(x) for x in c_idx]
+        term = coef * np.transpose(arrays[name], perm)
+        out = term if out is None else out + term
+    return np.ascontiguousarray(out)
+
+
+def termlist_run(statements, arrays):
+    """Runs statements in order; ``statements`` are (lhs name, kind, c_idx,
+    terms) tuples; ``arrays`` maps input names to numpy arrays and receives
+    every assigned array."""
+    for lhs, kind, c_idx, terms in statements:
+        arrays[lhs] = termlist_statement(kind, c_idx, terms, arrays)
+    return arrays

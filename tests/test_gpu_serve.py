@@ -1,0 +1,204 @@
+"""Serving runtime on the GPU: every statement of the UHF T1/T2 lists against
+the einsum oracle, mixed precision against FP64, bitwise identity of a
+request's results alone vs in a mix, and budget admission."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from dtcf_amd import mixed, termlist
+from dtcf_amd import serve as S
+from dtcf_amd._lib import lib
+
+pytestmark = pytest.mark.gpu
+
+ASYM = {"Oa": 3, "Ob": 2, "Va": 5, "Vb": 4}
+
+
+@pytest.fixture(scope="module")
+def lists():
+    return {"t1": termlist.load("t1"), "t2": termlist.load("t2")}
+
+
+def oracle_statements(tl):
+    out = []
+    for s in tl.statements:
+        if s.kind == "mul":
+            out.append((s.lhs.name, "mul", s.lhs.idx, [(t.name, t.idx) for t in s.terms]))
+        else:
+            out.append((s.lhs.name, "sum", s.lhs.idx, [(t.name, t.idx, t.coef) for t in s.terms]))
+    return out
+
+
+def oracle_run(tl, system):
+    arrays = {n: system.host(n) for n in tl.inputs()}
+    return oracle.termlist_run(oracle_statements(tl), arrays)
+
+
+@pytest.mark.parametrize("which", ["t1", "t2"])
+def test_every_statement_matches_oracle(dev, lists, which):
+    tl = lists[which]
+    print(f"serve parity {which} sizes {ASYM}", flush=True)
+    with S.Server(1 << 30, nslots=4) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=ASYM)
+        prog = S.Program(srv, tl, sysm, expose=True)
+        assert len(prog.outputs) == len(tl.statements)
+        got = prog.run()
+        ref = oracle_run(tl, sysm)
+        worst = 0.0
+        for name in prog.outputs:
+            r = ref[name]
+            scale = max(np.abs(r).max(), 1e-300)
+            err = np.abs(got[name] - r).max() / scale
+            worst = max(worst, err)
+            assert err <= 1e-12, (name, err)
+        print(f"  {len(prog.outputs)} arrays, worst relative error {worst:.2e}")
+        prog.destroy()
+
+
+@pytest.mark.parametrize("which", ["t1", "t2"])
+def test_concurrent_program_matches_oracle(dev, lists, which):
+    """Temporaries share the request block; with 8 slots statements finish out
+    of program order, so a reused region must wait for every reader of its
+    previous occupant."""
+    tl = lists[which]
+    sizes = {"Oa": 4, "Ob": 3, "Va": 12, "Vb": 10}
+    print(f"serve concurrent {which} sizes {sizes}", flush=True)
+    with S.Server(1 << 30, nslots=8) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+        prog = S.Program(srv, tl, sysm)
+        ref = oracle_run(tl, sysm)
+        for rep in range(3):
+            got = prog.run()
+            for name in prog.outputs:
+                err = np.abs(got[name] - ref[name]).max() / np.abs(ref[name]).max()
+                assert err <= 1e-12, (rep, name, err)
+        prog.destroy()
+
+
+@pytest.mark.parametrize("prec,ns,tol", [(mixed.I8, 5, 5e-8), (mixed.F32, 1, 5e-5),
+                                         (mixed.BF16X3, 3, 5e-5)])
+def test_mixed_program_against_fp64(dev, lists, prec, ns, tol):
+    tl = lists["t2"]
+    sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 10}
+    print(f"serve mixed prec={prec} ns={ns}", flush=True)
+    with S.Server(1 << 30, nslots=4) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+        p64 = S.Program(srv, tl, sysm)
+        pmx = S.Program(srv, tl, sysm, prec=prec, nslice=ns)
+        r64, rmx = p64.run(), pmx.run()
+        for name in p64.outputs:
+            err = np.abs(rmx[name] - r64[name]).max() / np.abs(r64[name]).max()
+            print(f"  {name}: max |dR| / max |R| = {err:.2e}")
+            assert err <= tol, (name, err)
+        p64.destroy()
+        pmx.destroy()
+
+
+def _outputs_host(outs):
+    return [o.cpu().numpy().copy() for o in outs]
+
+
+def test_alone_vs_mix_bitwise(dev, lists):
+    t1, t2 = lists["t1"], lists["t2"]
+    sa = {"Oa": 6, "Ob": 5, "Va": 20, "Vb": 18}
+    sb = {"Oa": 4, "Ob": 4, "Va": 14, "Vb": 14}
+    print("serve alone-vs-mix", flush=True)
+    with S.Server(4 << 30, nslots=6) as srv:
+        A = S.System(srv, 0, 0, [t1, t2], sizes=sa, seed=11)
+        B = S.System(srv, 0, 0, [t1, t2], sizes=sb, seed=12)
+        progs = {
+            "A2": S.Program(srv, t2, A), "A1": S.Program(srv, t1, A),
+            "B2": S.Program(srv, t2, B), "B1": S.Program(srv, t1, B),
+            "A2i8": S.Program(srv, t2, A, prec=mixed.I8, nslice=5),
+        }
+        torch.cuda.synchronize()
+        alone = {}
+        for k in ("A2", "A2i8", "B1"):
+            rid, outs = progs[k].submit()
+            assert srv.wait(rid, 120000) == 0
+            srv.release(rid)
+            alone[k] = _outputs_host(outs)
+        # the same requests again, interleaved with others
+        order = ["B2", "A2", "A1", "B1", "A2i8", "B2", "A1", "A2", "B1", "A2i8"]
+        subs = [(k,) + progs[k].submit() for k in order]
+        for k, rid, outs in subs:
+            assert srv.wait(rid, 120000) == 0
+        for k, rid, outs in subs:
+            if k in alone:
+                for x, y in zip(_outputs_host(outs), alone[k]):
+                    assert np.array_equal(x, y), k
+            srv.release(rid)
+        for p in progs.values():
+            p.destroy()
+
+
+def test_results_independent_of_slot_count(dev, lists):
+    tl = lists["t2"]
+    sizes = {"Oa": 5, "Ob": 4, "Va": 16, "Vb": 12}
+    res = []
+    for nslots in (1, 8):
+        with S.Server(1 << 30, nslots=nslots) as srv:
+            sysm = S.System(srv, 0, 0, [tl], sizes=sizes, seed=3)
+            p = S.Program(srv, tl, sysm)
+            res.append(p.run())
+            p.destroy()
+    for n in res[0]:
+        assert np.array_equal(res[0][n], res[1][n]), n
+
+
+def test_budget_admission_and_rejection(dev, lists):
+    tl = lists["t2"]
+    sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 12}
+    with S.Server(1 << 30, nslots=4) as probe:
+        ps = S.System(probe, 0, 0, [tl], sizes=sizes)
+        pp = S.Program(probe, tl, ps)
+        block, resident = pp.block_bytes, probe.usage()["resident"]
+        pp.destroy()
+    # room for the inputs and one and a half request blocks
+    budget = resident + 4096 + block * 3 // 2
+    with S.Server(budget, nslots=4) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+        p = S.Program(srv, tl, sysm)
+        torch.cuda.synchronize()
+        subs = [p.submit() for _ in range(4)]
+        for rid, _ in subs:
+            assert srv.wait(rid, 120000) == 0
+        u = srv.usage()
+        assert u["peak_reserved"] <= ((block + 255) // 256) * 256
+        infos = [srv.info(rid) for rid, _ in subs]
+        # one at a time: each admitted after the previous finished
+        for a, b in zip(infos, infos[1:]):
+            assert b.t_admit >= a.t_done
+        first = [x.cpu().numpy() for x in subs[0][1]]
+        for _, outs in subs[1:]:
+            for x, y in zip(outs, first):
+                assert np.array_equal(x.cpu().numpy(), y)
+        for rid, _ in subs:
+            srv.release(rid)
+        p.destroy()
+    # a request block that cannot fit beside the resident inputs is refused
+    with S.Server(resident + 4096 + block // 2, nslots=2) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+        p = S.Program(srv, tl, sysm)
+        with pytest.raises(RuntimeError, match="budget"):
+            p.submit()
+        p.destroy()
+
+
+def test_task_timestamps_ordered(dev, lists):
+    tl = lists["t1"]
+    with S.Server(1 << 28, nslots=3) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=ASYM)
+        p = S.Program(srv, tl, sysm)
+        torch.cuda.synchronize()
+        rid, _ = p.submit()
+        assert srv.wait(rid, 120000) == 0
+        inf = srv.info(rid)
+        assert inf.state == 2 and inf.ntasks == len(tl.statements)
+        assert inf.t_submit <= inf.t_admit <= inf.t_done
+        r, i, d = srv.task_times(rid, inf.ntasks)
+        assert np.all(r >= inf.t_admit) and np.all(r <= i) and np.all(i <= d)
+        assert np.all(d <= inf.t_done)
+        srv.release(rid)
+        p.destroy()

@@ -1,0 +1,106 @@
+"""UHF term-list front end: parsing, flop cross-check against the files' own
+flop lines and TOTAL COST, and the einsum oracle against brute-force loops."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from dtcf_amd import termlist as T
+
+
+@pytest.fixture(scope="module")
+def lists():
+    return {"t1": T.load("t1"), "t2": T.load("t2")}
+
+
+def test_shipped_lists_match_reference_files():
+    for which, ref in (("t1", "_uhf_t1.eq"), ("t2", "_uhf_t2.eq")):
+        path = os.path.join("/root/reference", ref)
+        if not os.path.exists(path):
+            pytest.skip("reference tree not present")
+        with open(path, "rb") as f, open(T.data_path(f"uhf_{which}.eq"), "rb") as g:
+            assert f.read() == g.read()
+
+
+def test_statement_counts(lists):
+    t1, t2 = lists["t1"], lists["t2"]
+    assert (len(t1.statements), len(t1.contractions), len(t1.sums)) == (72, 45, 27)
+    assert (len(t2.statements), len(t2.contractions), len(t2.sums)) == (306, 182, 124)
+    # the "53 + 231 multiplication terms": statements containing a '*'
+    star = lambda tl: sum(1 for s in tl.statements if s.kind == "mul" or any(t.scaled for t in s.terms))
+    assert (star(t1), star(t2)) == (53, 231)
+    assert t1.outputs() == ["ra_vo", "rb_vo"]
+    assert sorted(t2.outputs()) == ["raa_vvoo", "rab_vvoo", "rbb_vvoo"]
+    assert t1.ranges == {"Oa": 10, "Ob": 10, "Va": 100, "Vb": 100}
+
+
+@pytest.mark.parametrize("which", ["t1", "t2"])
+def test_flops_match_file(lists, which):
+    tl = lists[which]
+    bad, tot, total_cost, line_sum, badf = T.check_flops(tl)
+    assert badf == []                       # every formula evaluates to its number
+    assert line_sum == total_cost           # lines add up to TOTAL COST
+    assert {s.lhs.name for s, _, _ in bad} == T.GENERATOR_EXCEPTIONS[which]
+    fixed = tot + sum(f - g for _, g, f in bad)
+    assert fixed == total_cost
+    assert total_cost == {"t1": 856619400.0, "t2": 95288817300.0}[which]
+
+
+def test_flop_rules_on_known_lines(lists):
+    t2 = lists["t2"]
+    by = {s.lhs.name: s for s in t2.statements}
+    # contracted antisymmetric pair in both operands: n(n+1)/2 (_uhf_t2.eq:11-12)
+    assert by["_a14811"].gen_flops() == 10 * 10 * 10 * (101 * 100 / 2) * 2
+    # both pairs packed (_a3645: vvvv * T2)
+    assert by["_a3645"].gen_flops() == (101 * 100 / 2) * (11 * 10 / 2) * (101 * 100 / 2) * 2
+    # dense flops ignore antisymmetry
+    assert by["_a3645"].dense_flops() == 2 * 100 ** 4 * 10 ** 2
+
+
+def test_parser_rejects_bad_input():
+    src = """{
+ range O = 4.0;
+ index h1, h2 = O;
+ array x([O][O]), y([O][O]);
+ x[h1,h2] = (y[h1] * y[h2,h1])
+}"""
+    with pytest.raises(ValueError):
+        T.parse(src)
+
+
+def _loops_mul(c_idx, a_idx, A, b_idx, B, ext):
+    idx = list(dict.fromkeys(list(c_idx) + list(a_idx) + list(b_idx)))
+    out = np.zeros([ext[x] for x in c_idx])
+    for vals in itertools.product(*[range(ext[x]) for x in idx]):
+        v = dict(zip(idx, vals))
+        out[tuple(v[x] for x in c_idx)] += A[tuple(v[x] for x in a_idx)] * B[tuple(v[x] for x in b_idx)]
+    return out
+
+
+def test_einsum_oracle_against_loops():
+    rng = np.random.default_rng(3)
+    ext = {"p1": 3, "p2": 4, "h1": 2, "h2": 5, "p3": 3}
+    A = rng.standard_normal((3, 2, 3, 5))      # [p1, h1, p3, h2]
+    B = rng.standard_normal((3, 4))            # [p3, p2]
+    c_idx, a_idx, b_idx = ("p2", "h1", "p1", "h2"), ("p1", "h1", "p3", "h2"), ("p3", "p2")
+    got = oracle.termlist_statement("mul", c_idx, [("A", a_idx), ("B", b_idx)], {"A": A, "B": B})
+    ref = _loops_mul(c_idx, a_idx, A, b_idx, B, ext)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-13)
+    # outer product (no contracted index)
+    X, Y = rng.standard_normal((3, 2)), rng.standard_normal((4, 5))
+    got = oracle.termlist_statement("mul", ("p2", "p1", "h1", "h2"),
+                                    [("X", ("p1", "h1")), ("Y", ("p2", "h2"))], {"X": X, "Y": Y})
+    ref = _loops_mul(("p2", "p1", "h1", "h2"), ("p1", "h1"), X, ("p2", "h2"), Y, ext)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-13)
+    # permuted sum
+    P = rng.standard_normal((3, 4, 2, 5))      # [p1, p2, h1, h2]
+    Q = rng.standard_normal((4, 3, 5, 2))      # [p2, p1, h2, h1]
+    got = oracle.termlist_statement("sum", ("p1", "p2", "h1", "h2"),
+                                    [("P", ("p1", "p2", "h1", "h2"), 1.0),
+                                     ("Q", ("p2", "p1", "h2", "h1"), -0.5)], {"P": P, "Q": Q})
+    ref = np.zeros((3, 4, 2, 5))
+    for a, b, i, j in itertools.product(range(3), range(4), range(2), range(5)):
+        ref[a, b, i, j] = P[a, b, i, j] - 0.5 * Q[b, a, j, i]
+    np.testing.assert_allclose(got, ref, rtol=1e-14, atol=1e-14)
