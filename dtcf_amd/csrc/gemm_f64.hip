@@ -472,17 +472,26 @@ int launch_one(GemmArgs g, hipStream_t s)
     g.t_full = T;
     g.sk = 1;
     g.kchunk = nk;
+    // Under-filled grid (at most half the slots): every tile is cut into K
+    // slices.  Either way rem * sk <= slots, and every slice is non-empty.
+    uint32_t nsplit = 0, sk = 0;
     if (T > slots && rem > 0 && rem <= slots / 2 && nk >= 32) {
-        const uint32_t sk = std::min<uint32_t>(slots / rem, (uint32_t)(nk / 8));
-        if (sk >= 2) {
-            double *ws = nullptr;
-            st = split_workspace(s, (size_t)rem * sk * BM * BN * sizeof(double), &ws);
-            if (st != DTCF_OK) return st;
-            g.ws = ws;
-            g.t_full = T - rem;
-            g.sk = sk;
-            g.kchunk = (nk + sk - 1) / sk;
-        }
+        nsplit = rem;
+        sk = std::min<uint32_t>(slots / rem, (uint32_t)(nk / 8));
+    } else if (T <= slots / 2 && nk >= 16) {
+        nsplit = T;
+        sk = std::min<uint32_t>(slots / T, (uint32_t)(nk / 4));
+    }
+    if (sk >= 2) {
+        const int64_t kchunk = (nk + sk - 1) / sk;
+        sk = (uint32_t)((nk + kchunk - 1) / kchunk);
+        double *ws = nullptr;
+        st = split_workspace(s, (size_t)nsplit * sk * BM * BN * sizeof(double), &ws);
+        if (st != DTCF_OK) return st;
+        g.ws = ws;
+        g.t_full = T - nsplit;
+        g.sk = sk;
+        g.kchunk = kchunk;
     }
     const unsigned grid = g.t_full + (T - g.t_full) * g.sk;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, g);

@@ -207,6 +207,10 @@ struct PStmt {
     int64_t ldc = 0;
     int64_t scratch_off = -1, scratch_bytes = 0;
     int64_t sa_bytes = 0, sb_bytes = 0, ea_off = 0, eb_off = 0;
+    // FP64: operands in GEMM position 0 (A) / 1 (B) packed into scratch first
+    bool pack[2] = {false, false};
+    dtcf_strided pdesc[2];
+    int64_t poff[2] = {0, 0};
     // sum
     std::vector<int> terms;
     std::vector<double> coefs;
@@ -387,8 +391,47 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
     s.m = (int64_t)Ar.size();
     s.n = (int64_t)Br.size();
     s.k = (int64_t)Ak.size();
-    int st = dtcf_view_create(s.m, s.k, Ar.data(), Ak.data(), nullptr, s.m, s.k, &s.av);
-    if (st == DTCF_OK) st = dtcf_view_create(s.n, s.k, Br.data(), Bk.data(), nullptr, s.n, s.k, &s.bv);
+    // FP64: an operand with neither direction contiguous is permuted into a
+    // dense [rows][K] copy first (one pass at HBM speed) unless it is small.
+    // Mixed precision re-lays operands out in its split pass anyway.
+    int64_t scr = 0;
+    std::vector<int64_t> dense_r[2], dense_k[2];
+    for (int q = 0; q < 2 && p->prec == 0; q++) {
+        const auto &R = q == 0 ? Ar : Br;
+        const auto &Kk = q == 0 ? Ak : Bk;
+        const int64_t rows = (int64_t)R.size(), K = (int64_t)Kk.size();
+        if (opscore(R, Kk) > 0.0 || rows * K < (int64_t(1) << 16)) continue;
+        const bool opa = (q == 0) != s.swap;  // operand A of the statement?
+        const PArr &X = opa ? A : B;
+        const auto &xi = opa ? ai : bi;
+        const auto &rnames = opa ? ea : eb;
+        std::vector<std::string> names(rnames);
+        names.insert(names.end(), best.korder.begin(), best.korder.end());
+        DTCF_REQUIRE(names.size() == xi.size() && names.size() <= DTCF_MAX_DIMS, "program: pack rank");
+        dtcf_strided &d = s.pdesc[q];
+        d = dtcf_strided{};
+        d.dims = (int32_t)names.size();
+        int64_t ostride = 1;
+        for (int j = (int)names.size() - 1; j >= 0; j--) {
+            int pos = -1;
+            index_map(xi, names[j], &pos);
+            d.ext[j] = X.dims[pos];
+            d.istr[j] = X.stride[pos];
+            d.ostr[j] = ostride;
+            ostride *= X.dims[pos];
+        }
+        s.pack[q] = true;
+        s.poff[q] = scr;
+        scr += granule(rows * K * (int64_t)sizeof(double));
+        dense_r[q].resize(rows);
+        dense_k[q].resize(K);
+        for (int64_t r = 0; r < rows; r++) dense_r[q][r] = r * K;
+        for (int64_t c = 0; c < K; c++) dense_k[q][c] = c;
+    }
+    const auto &Ar2 = s.pack[0] ? dense_r[0] : Ar, &Ak2 = s.pack[0] ? dense_k[0] : Ak;
+    const auto &Br2 = s.pack[1] ? dense_r[1] : Br, &Bk2 = s.pack[1] ? dense_k[1] : Bk;
+    int st = dtcf_view_create(s.m, s.k, Ar2.data(), Ak2.data(), nullptr, s.m, s.k, &s.av);
+    if (st == DTCF_OK) st = dtcf_view_create(s.n, s.k, Br2.data(), Bk2.data(), nullptr, s.n, s.k, &s.bv);
     if (st != DTCF_OK) return st;
     bool dense = true;
     for (int64_t r = 0; dense && r < s.m; r++) dense = Cr[r] == r * s.n;
@@ -399,6 +442,7 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
         st = dtcf_view_create(s.m, s.n, Cr.data(), Cc.data(), nullptr, s.m, s.n, &s.cv);
         if (st != DTCF_OK) return st;
     }
+    s.scratch_bytes = scr;
     s.flops = 2.0 * (double)s.m * (double)s.n * (double)s.k;
     if (p->prec != 0) {
         s.sa_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.m, s.k));
@@ -446,8 +490,19 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
     DTCF_HIP(hipGetLastError());
     const double *A = arr_ptr(p, p->arrs[s.a], r), *B = arr_ptr(p, p->arrs[s.b], r);
     if (s.swap) std::swap(A, B);
-    if (p->prec == 0)
-        return dtcf_gemm_view_f64(s.m, s.n, s.k, s.alpha, A, s.av, B, s.bv, c, s.cv, s.ldc, stream);
+    if (p->prec == 0) {
+        const double *op[2] = {A, B};
+        for (int q = 0; q < 2; q++) {
+            if (!s.pack[q]) continue;
+            double *dst = reinterpret_cast<double *>(r->base + s.scratch_off + s.poff[q]);
+            int st = dtcf_permute_copy_f64(op[q], dst, &s.pdesc[q], 1, nullptr, 0, nullptr, 0, 1.0,
+                                           0.0, stream);
+            if (st != DTCF_OK) return st;
+            op[q] = dst;
+        }
+        return dtcf_gemm_view_f64(s.m, s.n, s.k, s.alpha, op[0], s.av, op[1], s.bv, c, s.cv, s.ldc,
+                                  stream);
+    }
     char *scr = r->base + s.scratch_off;
     int32_t *ea = reinterpret_cast<int32_t *>(scr + s.ea_off);
     int32_t *eb = reinterpret_cast<int32_t *>(scr + s.eb_off);
