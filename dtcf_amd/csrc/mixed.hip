@@ -348,7 +348,235 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
         }
 }
 
+// ------------------------------------------------------ fused INT8 digits
+// All digit products of one C tile in one kernel.  Per K step of 64 bytes
+// the workgroup stages every digit plane of its A rows and B rows in LDS
+// once and runs every product (i, j), i + j <= L, from registers.  Products
+// of one level l = i + j share the scale 2^(ea + eb - 7l), so they sum
+// exactly in one INT32 accumulator per level: L-1 accumulator sets instead
+// of one FP64 read-modify-write of C per product.  A level holds at most
+// P = min(L-1, NS) products, so INT32 stays exact for P * 127^2 * Kc < 2^31;
+// every Kc bytes of K (a multiple of 64) the levels are flushed into C.
+//
+// Workgroup: 256 threads, C tile 64 x 128, waves 2 x 2 each owning 32 x 64
+// as 2 x 4 MFMA 16x16x64 blocks.  LDS: two stages of 12 * NS fragments of
+// 1 KiB ([operand digit][16-row block][lane][16 B], the fragment-major order
+// of the per-product kernel: lane = row % 16 + 16 * kq holds bytes
+// 16kq..16kq+15 of the 64-byte step of its row).  Each fragment is one
+// direct global->LDS load of 64 lanes x 16 B, issued a step ahead.
+constexpr int OZ_BM = 64, OZ_BN = 128, OZ_BK = 64;
+
+struct OzArgs {
+    const unsigned char *A, *B;  // digit planes [NS][Mp][Kp], [NS][Np][Kp]
+    const int32_t *ea, *eb;
+    double alpha;
+    int64_t M, N, Kp, Mp, Np;
+    int64_t kchunk_steps;        // K steps between INT32 flushes
+    double *C;
+    ViewArgs cv;
+    int64_t ldc;
+    uint32_t tiles_m, tiles_n;
+};
+
+template <int NS, int L, bool CVIEW>
+__device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][2][4], int64_t m0, int64_t n0,
+                                int wm, int wn, int lane)
+{
+    const int fr = lane & 15, fq = lane >> 4;
+    // row offsets and exponents once per flush (8 rows per lane)
+    int64_t rof[2][4];
+    int er[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t row = m0 + wm * 32 + i * 16 + 4 * fq + r;
+            const bool ok = row < g.M;
+            er[i][r] = ok ? g.ea[row] : 0;
+            if constexpr (CVIEW) rof[i][r] = ok ? g.cv.rofs[row] : 0;
+            else rof[i][r] = row * g.ldc;
+        }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int64_t col = n0 + wn * 64 + j * 16 + fr;
+        if (col >= g.N) continue;
+        const int ebc = g.eb[col];
+        int64_t cof;
+        if constexpr (CVIEW) cof = g.cv.cofs[col];
+        else cof = col;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wm * 32 + i * 16 + 4 * fq + r;
+                if (row >= g.M) continue;
+                const int e = er[i][r] + ebc;
+                double v = 0.0;
+#pragma unroll
+                for (int l = L - 2; l >= 0; l--)  // smallest terms first
+                    v += ldexp((double)acc[l][i][j][r], e - 7 * (l + 2));
+                int64_t addr = rof[i][r] + cof;
+                if constexpr (CVIEW) {
+                    if (g.cv.blk) addr = vaddr(g.cv, row, col);
+                }
+                g.C[addr] += g.alpha * v;
+            }
+    }
+}
+
+template <int NS, int L, bool CVIEW>
+__global__ __launch_bounds__(256, 1) void ozaki_kernel(OzArgs g)
+{
+    constexpr int FA = NS * (OZ_BM / 16), FB = NS * (OZ_BN / 16);
+    constexpr int NF = FA + FB;   // fragments per stage
+    constexpr int PER = NF / 4;   // per wave
+    static_assert(NF % 4 == 0, "fragments split evenly over 4 waves");
+    // Two stages as two objects: the direct loads into one stage provably do
+    // not alias the fragment reads of the other, so no wait is placed
+    // between them.
+    __shared__ i4 stage0[NF * 64];
+    __shared__ i4 stage1[NF * 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    // tile: XCD-aware remap, then groups of 8 tile rows
+    const uint32_t T = g.tiles_m * g.tiles_n;
+    uint32_t q = blockIdx.x;
+    if ((T & 7u) == 0) q = (q & 7u) * (T >> 3) + (q >> 3);
+    const uint32_t width = 8 * g.tiles_n;
+    const uint32_t first = (q / width) * 8;
+    const uint32_t gsz = min(g.tiles_m - first, 8u);
+    const uint32_t tm = first + (q % width) % gsz, tn = (q % width) / gsz;
+    const int64_t m0 = (int64_t)tm * OZ_BM, n0 = (int64_t)tn * OZ_BN;
+
+    const size_t pa = (size_t)g.Mp * g.Kp, pb = (size_t)g.Np * g.Kp;
+    // Fragment f = wave * PER + c comes from a wave-uniform base (scalar
+    // registers) plus one per-lane offset; it is loaded straight into LDS
+    // (16 bytes per lane, lane l at byte 16 l of the fragment).
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int64_t lane_off = (int64_t)(lane & 15) * g.Kp + 16 * (lane >> 4);
+    auto frag_base = [&](int c) __attribute__((always_inline)) -> const unsigned char * {
+        const int f = wv * PER + c;
+        if (f < FA) {
+            const int d = f / (OZ_BM / 16), b = f % (OZ_BM / 16);
+            return g.A + d * pa + (m0 + 16 * b) * g.Kp;
+        }
+        const int h = f - FA, d = h / (OZ_BN / 16), b = h % (OZ_BN / 16);
+        return g.B + d * pb + (n0 + 16 * b) * g.Kp;
+    };
+    auto issue = [&](i4 *stage, int64_t kt) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < PER; c++)
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(frag_base(c) + lane_off + kt * OZ_BK),
+                (__attribute__((address_space(3))) void *)(stage + (wv * PER + c) * 64), 16, 0, 0);
+    };
+
+    i4 acc[L - 1][2][4];
+#pragma unroll
+    for (int l = 0; l < L - 1; l++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+
+    auto step = [&](const i4 *st) __attribute__((always_inline)) {
+        i4 af[NS][2];
+#pragma unroll
+        for (int d = 0; d < NS; d++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) af[d][i] = st[(d * (OZ_BM / 16) + wm * 2 + i) * 64 + lane];
+        // B digit by digit (the next one read while this one is used); for
+        // digit dj every product (di, dj), di + dj <= L, over all 8 blocks.
+        // Two updates of one accumulator -- (di, dj) and (di - 1, dj + 1) --
+        // are about 8 * (NS - 1) MFMAs apart: enough independent chains to
+        // keep the INT8 MFMA pipe busy from one wave.
+        i4 bcur[4], bnxt[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) bcur[j] = st[(FA + wn * 4 + j) * 64 + lane];
+#pragma unroll
+        for (int dj = 1; dj <= NS; dj++) {
+            if (dj < NS) {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    bnxt[j] = st[(FA + dj * (OZ_BN / 16) + wn * 4 + j) * 64 + lane];
+            }
+#pragma unroll
+            for (int di = 1; di <= NS; di++) {
+                if (di + dj > L) continue;
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[di + dj - 2][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            af[di - 1][i], bcur[j], acc[di + dj - 2][i][j], 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) bcur[j] = bnxt[j];
+        }
+    };
+    int64_t since = 0;
+    auto after = [&](bool more) __attribute__((always_inline)) {
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads have landed
+        __syncthreads();                // ... and everyone's; stage reads done
+        if (++since == g.kchunk_steps && more) {
+            oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+#pragma unroll
+            for (int l = 0; l < L - 1; l++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+            since = 0;
+        }
+    };
+
+    const int64_t nk = g.Kp / OZ_BK;
+    issue(stage0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; kt += 2) {
+        if (kt + 1 < nk) issue(stage1, kt + 1);
+        step(stage0);
+        after(kt + 1 < nk);
+        if (kt + 1 >= nk) break;
+        if (kt + 2 < nk) issue(stage0, kt + 2);
+        step(stage1);
+        after(kt + 2 < nk);
+    }
+    oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+}
+
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// 1: digit products of INT8 splits run in the fused kernel where a
+// specialisation exists; 0: one launch per product.
+int g_fused_i8 = 1;
+
+template <int NS, int L>
+int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
+{
+    const unsigned grid = g.tiles_m * g.tiles_n;
+    if (cview) hipLaunchKernelGGL((ozaki_kernel<NS, L, true>), dim3(grid), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(256), 0, s, g);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+// Runs the fused kernel when (ns, lev) has a specialisation; *done = false
+// otherwise.
+int oz_dispatch(int ns, int lev, const OzArgs &g, bool cview, hipStream_t s, bool *done)
+{
+    *done = true;
+    switch (ns * 16 + lev) {
+    case 3 * 16 + 4: return oz_launch<3, 4>(g, cview, s);
+    case 4 * 16 + 4: return oz_launch<4, 4>(g, cview, s);
+    case 4 * 16 + 5: return oz_launch<4, 5>(g, cview, s);
+    case 5 * 16 + 5: return oz_launch<5, 5>(g, cview, s);
+    case 5 * 16 + 6: return oz_launch<5, 6>(g, cview, s);
+    default: *done = false; return DTCF_OK;
+    }
+}
 
 }  // namespace
 }  // namespace dtcf
@@ -457,6 +685,12 @@ static void lp_launch(const LPArgs &g, hipStream_t s)
     hipLaunchKernelGGL((lp_gemm_kernel<PREC, CV>), dim3(grid), dim3(256), 0, s, g);
 }
 
+extern "C" int dtcf_gemm_split_set_fused(int on)
+{
+    g_fused_i8 = on ? 1 : 0;
+    return DTCF_OK;
+}
+
 extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k,
                                    double alpha, const void *A_slices, const int32_t *ea,
                                    const void *B_slices, const int32_t *eb, int max_level,
@@ -470,7 +704,7 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
     if (scaled) DTCF_REQUIRE(ea && eb, "gemm_split: scaled slices need row exponents");
     const int ns = prec == DTCF_PREC_F32 ? 1 : prec == DTCF_PREC_I8 ? nslice : 3;
     DTCF_REQUIRE(ns >= 1 && ns <= 7, "gemm_split: bad slice count");
-    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(k < 133144, "gemm_split: INT8 products exact only for K < 133144");
+
     LPArgs g{};
     g.M = m;
     g.N = n;
@@ -494,6 +728,33 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
     // Slice pairs (i, j), 1-based, with i + j <= max_level (default ns + 1).
     const int lev = max_level > 0 ? max_level : ns + 1;
     hipStream_t s = as_stream(stream);
+    if (prec == DTCF_PREC_I8 && g_fused_i8) {
+        OzArgs o{};
+        o.A = (const unsigned char *)A_slices;
+        o.B = (const unsigned char *)B_slices;
+        o.ea = ea;
+        o.eb = eb;
+        o.alpha = alpha;
+        o.M = m;
+        o.N = n;
+        o.Kp = g.Kp;
+        o.Mp = g.Mp;
+        o.Np = g.Np;
+        o.C = C;
+        o.cv = g.cv;
+        o.ldc = ldc;
+        o.tiles_m = (uint32_t)(g.Mp / OZ_BM);
+        o.tiles_n = (uint32_t)(g.Np / OZ_BN);
+        // INT32 exact while (products per level) * 127^2 * K < 2^31
+        const int per_level = std::min(lev - 1, ns);
+        const int64_t kmax = ((int64_t(1) << 31) - 1) / ((int64_t)per_level * 127 * 127);
+        o.kchunk_steps = std::max<int64_t>(1, kmax / OZ_BK);
+        bool done = false;
+        int st = oz_dispatch(ns, lev, o, cv != nullptr, s, &done);
+        if (st != DTCF_OK || done) return st;
+    }
+    if (prec == DTCF_PREC_I8)
+        DTCF_REQUIRE(k < 133144, "gemm_split: INT8 products exact only for K < 133144");
     for (int i = 1; i <= ns; i++)
         for (int j = 1; j <= ns && i + j <= lev; j++) {
             g.A = (const unsigned char *)A_slices + (i - 1) * pa;

@@ -21,7 +21,9 @@
 // Issue.  A scheduler thread keeps `nslots` non-blocking streams.  Each
 // statement of an admitted request becomes ready when all its dependencies
 // have completed (observed with hipEventQuery on the host); ready statements
-// go to free slots oldest request first, then in program order.  A
+// go to free slots from the request with the least total work first (then
+// arrival order, then program order), so short requests are not queued
+// behind long ones.  A
 // statement runs entirely on one stream with kernels whose reduction order
 // is fixed by the shapes alone, so a request's results are bitwise the same
 // alone and beside other requests.
@@ -34,6 +36,7 @@
 #include <set>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -272,8 +275,9 @@ struct dtcf_server_s {
     std::deque<Req *> waiting;
     std::vector<Req *> active;
     std::map<int64_t, Req *> reqs;
-    std::set<std::pair<int64_t, int>> ready;  // (request seq, statement)
-    std::map<int64_t, Req *> byseq;
+    // Ready statements keyed (request flops, request id, statement):
+    // requests with less work first, then arrival order (shortest job first).
+    std::set<std::tuple<double, int64_t, int>> ready;
     int64_t next_id = 1, next_seq = 1;
     std::chrono::steady_clock::time_point t0;
     std::string fatal;
@@ -524,8 +528,8 @@ void finish_request(dtcf_server srv, Req *r)
         r->off = -1;
     }
     r->p->refs--;
-    srv->ready.erase(srv->ready.lower_bound({r->seq, -1}), srv->ready.lower_bound({r->seq + 1, -1}));
-    srv->byseq.erase(r->seq);
+    srv->ready.erase(srv->ready.lower_bound({r->p->flops, r->id, -1}),
+                     srv->ready.lower_bound({r->p->flops, r->id + 1, -1}));
     srv->active.erase(std::find(srv->active.begin(), srv->active.end(), r));
     srv->done_cv.notify_all();
 }
@@ -557,7 +561,7 @@ bool sched_pass(dtcf_server srv)
             for (int j : r->p->st[i].succ)
                 if (--r->ndeps[j] == 0) {
                     r->t_ready[j] = t;
-                    srv->ready.insert({r->seq, j});
+                    srv->ready.insert({r->p->flops, r->id, j});
                 }
         }
         if (r->remaining == 0 || (r->status != DTCF_OK && r->inflight == 0)) finish_request(srv, r);
@@ -579,13 +583,12 @@ bool sched_pass(dtcf_server srv)
         srv->reserved += granule(r->p->block);
         srv->peak_reserved = std::max(srv->peak_reserved, srv->reserved);
         srv->active.push_back(r);
-        srv->byseq[r->seq] = r;
         const auto &st = r->p->st;
         for (size_t j = 0; j < st.size(); j++) {
             r->ndeps[j] = (int)st[j].deps.size();
             if (r->ndeps[j] == 0) {
                 r->t_ready[j] = r->t_admit;
-                srv->ready.insert({r->seq, (int)j});
+                srv->ready.insert({r->p->flops, r->id, (int)j});
             }
         }
         if (st.empty()) finish_request(srv, r);
@@ -596,8 +599,8 @@ bool sched_pass(dtcf_server srv)
     for (auto &sl : srv->slots) {
         if (sl.r || srv->ready.empty()) continue;
         auto it = srv->ready.begin();
-        Req *r = srv->byseq[it->first];
-        const int i = it->second;
+        Req *r = srv->reqs[std::get<1>(*it)];
+        const int i = std::get<2>(*it);
         srv->ready.erase(it);
         changed = true;
         if (r->status != DTCF_OK) {  // a failed request issues nothing more

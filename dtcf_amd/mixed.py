@@ -25,7 +25,8 @@ def _declare():
     L.dtcf_fill_normal_f64.argtypes = [vp, i64, C.c_uint64, C.c_double, i64, vp]
     L.dtcf_fill_uniform_f64.argtypes = [vp, i64, C.c_uint64, C.c_double, C.c_double, vp]
     L.dtcf_antisymmetrize4_f64.argtypes = [vp, vp, i64, i64, i64, i64, vp]
-    for f in ("dtcf_split_f64", "dtcf_gemm_split_f64", "dtcf_fill_normal_f64",
+    L.dtcf_gemm_split_set_fused.argtypes = [C.c_int]
+    for f in ("dtcf_split_f64", "dtcf_gemm_split_f64", "dtcf_fill_normal_f64", "dtcf_gemm_split_set_fused",
               "dtcf_fill_uniform_f64", "dtcf_antisymmetrize4_f64"):
         getattr(L, f).restype = C.c_int
 
@@ -39,6 +40,11 @@ def _s():
 
 def _p(t):
     return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def set_fused(on: bool):
+    """INT8 digit products in one fused kernel (default) or one GEMM each."""
+    check(lib().dtcf_gemm_split_set_fused(1 if on else 0))
 
 
 def fill_normal(n, seed, sigma=1.0, device="cuda", start=0):

@@ -327,6 +327,11 @@ int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view v, void *d
  * accumulators are converted to FP64 and rescaled in the epilogue.  A_slices
  * from dtcf_split_f64 over an m x k view, B_slices over an n x k view.  C is
  * a view (cv) or dense row-major with leading dimension ldc. */
+/* INT8 digit products: 1 (default) runs every product of a call in one
+ * fused kernel (level sums in INT32, flushed to C every <= 26,624 K) where
+ * a specialisation exists -- (nslice, max_level) in {(3,4), (4,4), (4,5),
+ * (5,5), (5,6)}; otherwise, or with 0, one GEMM per product (K < 133,144). */
+int dtcf_gemm_split_set_fused(int on);
 int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k, double alpha,
                         const void *A_slices, const int32_t *ea, const void *B_slices,
                         const int32_t *eb, int max_level, double *C, dtcf_view cv, int64_t ldc,
@@ -411,8 +416,9 @@ int dtcf_program_destroy(dtcf_program p);
  * stay valid until the request completes.  Requests are admitted in
  * arrival order as their block fits in the arena (later, smaller requests
  * may pass a blocked one, at most 8 times); the statements of all admitted
- * requests are issued to free slots as their inputs complete, oldest
- * request first.  A request's results do not depend on what runs beside it. */
+ * requests are issued to free slots as their inputs complete, from the
+ * request with the least total work first (shortest job first, then arrival
+ * order).  A request's results do not depend on what runs beside it. */
 int dtcf_request_submit(dtcf_server s, dtcf_program p, double *const *outputs, int64_t *id);
 /* DTCF_OK when complete, DTCF_ERR_NOTREADY on timeout, another status if
  * the request failed.  timeout_ms < 0 waits forever. */

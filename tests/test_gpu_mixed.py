@@ -98,3 +98,39 @@ def test_device_generators_match_host(dev):
     y = mixed.antisymmetrize4(x, (6, 6, 4, 4)).cpu().numpy().reshape(6, 6, 4, 4)
     assert np.allclose(y, synth.antisymmetrize(x.cpu().numpy().reshape(6, 6, 4, 4)), atol=1e-15,
                        rtol=1e-15)
+
+
+@pytest.mark.parametrize("ns,lev", [(5, 6), (4, 5), (5, 5), (3, 4), (4, 4)])
+@pytest.mark.parametrize("m,n,k,cview", [(200, 136, 520, False), (64, 300, 96, True),
+                                         (130, 260, 27000, False), (70, 130, 27000, True)])
+def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
+    """The fused kernel sums each level exactly in INT32 (flushing every
+    26,624 K at most) and adds the levels to C once; the per-product path adds
+    every product to C.  Same digits, so the results agree to FP64 rounding."""
+    rng = np.random.default_rng(m * n + k + ns)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    C0 = rng.standard_normal((m, n))
+    Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    sa = mixed.Split(mixed.I8, Ad, mixed.dense_view(m, k), nslice=ns)
+    sb = mixed.Split(mixed.I8, Bd, mixed.dense_view(n, k), nslice=ns)
+    # C stored transposed (column-major) when cview
+    cv = ops.View(m, n, np.arange(m), np.arange(n) * m) if cview else None
+    out = []
+    try:
+        for fused in (True, False):
+            mixed.set_fused(fused)
+            Cd = torch.from_numpy((C0.T if cview else C0).copy().ravel()).to(dev)
+            mixed.gemm_split(sa, sb, Cd, alpha=0.75, ldc=n, cview=cv, max_level=lev)
+            r = Cd.cpu().numpy()
+            out.append(r.reshape(n, m).T if cview else r.reshape(m, n))
+    finally:
+        mixed.set_fused(True)
+    ref = C0 + 0.75 * A @ B.T
+    d = relerr(out[0] - C0, out[1] - C0)
+    e = relerr(out[0] - C0, ref - C0)
+    print(f"ns={ns} L={lev} {m}x{n}x{k} cview={cview}: fused vs per-product {d:.1e}, vs FP64 {e:.1e}")
+    assert d <= 1e-13
+    # normwise error vs FP64, dominated by the dropped digits (ns) or, for
+    # L = ns, by the dropped level-(ns+1) products
+    assert e <= {(5, 6): 5e-9, (4, 5): 5e-7, (5, 5): 1e-6, (3, 4): 3e-5, (4, 4): 3e-5}[(ns, lev)]
