@@ -358,8 +358,7 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
 // P = min(L-1, NS) products, so INT32 stays exact for P * 127^2 * Kc < 2^31;
 // every Kc bytes of K (a multiple of 64) the levels are flushed into C.
 //
-// Workgroup: 256 threads, C tile 64 x 128, waves 2 x 2 each owning 32 x 64
-// as 2 x 4 MFMA 16x16x64 blocks.  LDS: two stages of 12 * NS fragments of
+// Workgroup: C tile 64 x 128 (wave layout below).  LDS: two stages of 12 * NS fragments of
 // 1 KiB ([operand digit][16-row block][lane][16 B], the fragment-major order
 // of the per-product kernel: lane = row % 16 + 16 * kq holds bytes
 // 16kq..16kq+15 of the 64-byte step of its row).  Each fragment is one
@@ -378,27 +377,21 @@ struct OzArgs {
     uint32_t tiles_m, tiles_n;
 };
 
+// Workgroup of 8 waves (2 per SIMD) as 2 x 4, each owning 32 x 32 of the
+// 64 x 128 tile as 2 x 2 MFMA blocks: under 200 registers per wave, so two
+// waves share a SIMD and hide each other's MFMA and LDS latency.  (Two
+// 64 x 64 workgroups per CU measured slower: more L2 and LDS traffic per
+// product.)
+constexpr int OZ_WAVES = 8, OZ_WM = 32, OZ_WN = 32;
+
 template <int NS, int L, bool CVIEW>
-__device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][2][4], int64_t m0, int64_t n0,
-                                int wm, int wn, int lane)
+__device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][2][2], int64_t m0,
+                                         int64_t n0, int wm, int wn, int lane)
 {
     const int fr = lane & 15, fq = lane >> 4;
-    // row offsets and exponents once per flush (8 rows per lane)
-    int64_t rof[2][4];
-    int er[2][4];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int64_t row = m0 + wm * 32 + i * 16 + 4 * fq + r;
-            const bool ok = row < g.M;
-            er[i][r] = ok ? g.ea[row] : 0;
-            if constexpr (CVIEW) rof[i][r] = ok ? g.cv.rofs[row] : 0;
-            else rof[i][r] = row * g.ldc;
-        }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int64_t col = n0 + wn * 64 + j * 16 + fr;
+    for (int j = 0; j < 2; j++) {
+        const int64_t col = n0 + wn * OZ_WN + j * 16 + fr;
         if (col >= g.N) continue;
         const int ebc = g.eb[col];
         int64_t cof;
@@ -408,36 +401,35 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][2][4]
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t row = m0 + wm * 32 + i * 16 + 4 * fq + r;
+                const int64_t row = m0 + wm * OZ_WM + i * 16 + 4 * fq + r;
                 if (row >= g.M) continue;
-                const int e = er[i][r] + ebc;
+                const int e = g.ea[row] + ebc;
                 double v = 0.0;
 #pragma unroll
                 for (int l = L - 2; l >= 0; l--)  // smallest terms first
                     v += ldexp((double)acc[l][i][j][r], e - 7 * (l + 2));
-                int64_t addr = rof[i][r] + cof;
-                if constexpr (CVIEW) {
-                    if (g.cv.blk) addr = vaddr(g.cv, row, col);
-                }
+                int64_t addr;
+                if constexpr (CVIEW) addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + cof;
+                else addr = row * g.ldc + cof;
                 g.C[addr] += g.alpha * v;
             }
     }
 }
 
 template <int NS, int L, bool CVIEW>
-__global__ __launch_bounds__(256, 1) void ozaki_kernel(OzArgs g)
+__global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
 {
     constexpr int FA = NS * (OZ_BM / 16), FB = NS * (OZ_BN / 16);
-    constexpr int NF = FA + FB;   // fragments per stage
-    constexpr int PER = NF / 4;   // per wave
-    static_assert(NF % 4 == 0, "fragments split evenly over 4 waves");
+    constexpr int NF = FA + FB;                          // fragments per stage
+    constexpr int PER = (NF + OZ_WAVES - 1) / OZ_WAVES;  // per wave (last may be short)
     // Two stages as two objects: the direct loads into one stage provably do
     // not alias the fragment reads of the other, so no wait is placed
     // between them.
     __shared__ i4 stage0[NF * 64];
     __shared__ i4 stage1[NF * 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wv / (OZ_BN / OZ_WN), wn = wv % (OZ_BN / OZ_WN);
 
     // tile: XCD-aware remap, then groups of 8 tile rows
     const uint32_t T = g.tiles_m * g.tiles_n;
@@ -453,10 +445,8 @@ __global__ __launch_bounds__(256, 1) void ozaki_kernel(OzArgs g)
     // Fragment f = wave * PER + c comes from a wave-uniform base (scalar
     // registers) plus one per-lane offset; it is loaded straight into LDS
     // (16 bytes per lane, lane l at byte 16 l of the fragment).
-    const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int64_t lane_off = (int64_t)(lane & 15) * g.Kp + 16 * (lane >> 4);
-    auto frag_base = [&](int c) __attribute__((always_inline)) -> const unsigned char * {
-        const int f = wv * PER + c;
+    auto frag_base = [&](int f) __attribute__((always_inline)) -> const unsigned char * {
         if (f < FA) {
             const int d = f / (OZ_BM / 16), b = f % (OZ_BM / 16);
             return g.A + d * pa + (m0 + 16 * b) * g.Kp;
@@ -466,40 +456,44 @@ __global__ __launch_bounds__(256, 1) void ozaki_kernel(OzArgs g)
     };
     auto issue = [&](i4 *stage, int64_t kt) __attribute__((always_inline)) {
 #pragma unroll
-        for (int c = 0; c < PER; c++)
+        for (int c = 0; c < PER; c++) {
+            const int f = wv * PER + c;
+            if (NF % OZ_WAVES != 0 && f >= NF) break;  // wave-uniform
             __builtin_amdgcn_global_load_lds(
-                (const void *)(frag_base(c) + lane_off + kt * OZ_BK),
-                (__attribute__((address_space(3))) void *)(stage + (wv * PER + c) * 64), 16, 0, 0);
+                (const void *)(frag_base(f) + lane_off + kt * OZ_BK),
+                (__attribute__((address_space(3))) void *)(stage + f * 64), 16, 0, 0);
+        }
     };
 
-    i4 acc[L - 1][2][4];
+    i4 acc[L - 1][2][2];
 #pragma unroll
     for (int l = 0; l < L - 1; l++)
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+            for (int j = 0; j < 2; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
 
     auto step = [&](const i4 *st) __attribute__((always_inline)) {
         i4 af[NS][2];
 #pragma unroll
         for (int d = 0; d < NS; d++)
 #pragma unroll
-            for (int i = 0; i < 2; i++) af[d][i] = st[(d * (OZ_BM / 16) + wm * 2 + i) * 64 + lane];
+            for (int i = 0; i < 2; i++)
+                af[d][i] = st[(d * (OZ_BM / 16) + wm * (OZ_WM / 16) + i) * 64 + lane];
         // B digit by digit (the next one read while this one is used); for
-        // digit dj every product (di, dj), di + dj <= L, over all 8 blocks.
+        // digit dj every product (di, dj), di + dj <= L, over the 4 blocks.
         // Two updates of one accumulator -- (di, dj) and (di - 1, dj + 1) --
-        // are about 8 * (NS - 1) MFMAs apart: enough independent chains to
-        // keep the INT8 MFMA pipe busy from one wave.
-        i4 bcur[4], bnxt[4];
+        // are about 4 * (NS - 1) MFMAs apart, and the SIMD's second wave
+        // interleaves its own chains.
+        i4 bcur[2], bnxt[2];
 #pragma unroll
-        for (int j = 0; j < 4; j++) bcur[j] = st[(FA + wn * 4 + j) * 64 + lane];
+        for (int j = 0; j < 2; j++) bcur[j] = st[(FA + wn * (OZ_WN / 16) + j) * 64 + lane];
 #pragma unroll
         for (int dj = 1; dj <= NS; dj++) {
             if (dj < NS) {
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    bnxt[j] = st[(FA + dj * (OZ_BN / 16) + wn * 4 + j) * 64 + lane];
+                for (int j = 0; j < 2; j++)
+                    bnxt[j] = st[(FA + dj * (OZ_BN / 16) + wn * (OZ_WN / 16) + j) * 64 + lane];
             }
 #pragma unroll
             for (int di = 1; di <= NS; di++) {
@@ -507,12 +501,12 @@ __global__ __launch_bounds__(256, 1) void ozaki_kernel(OzArgs g)
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++)
+                    for (int j = 0; j < 2; j++)
                         acc[di + dj - 2][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                             af[di - 1][i], bcur[j], acc[di + dj - 2][i][j], 0, 0, 0);
             }
 #pragma unroll
-            for (int j = 0; j < 4; j++) bcur[j] = bnxt[j];
+            for (int j = 0; j < 2; j++) bcur[j] = bnxt[j];
         }
     };
     int64_t since = 0;
@@ -526,7 +520,7 @@ __global__ __launch_bounds__(256, 1) void ozaki_kernel(OzArgs g)
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+                    for (int j = 0; j < 2; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
             since = 0;
         }
     };
@@ -557,8 +551,8 @@ template <int NS, int L>
 int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
 {
     const unsigned grid = g.tiles_m * g.tiles_n;
-    if (cview) hipLaunchKernelGGL((ozaki_kernel<NS, L, true>), dim3(grid), dim3(256), 0, s, g);
-    else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(256), 0, s, g);
+    if (cview) hipLaunchKernelGGL((ozaki_kernel<NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+    else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
     DTCF_HIP(hipGetLastError());
     return DTCF_OK;
 }
