@@ -178,8 +178,8 @@ def leg_blocksparse(dev, log, steps=5):
         out[name] = {"block_pairs": term.npairs, "flops_performed": term.flops, "ms": round(ms, 4),
                      "tflops": round(term.flops / ms / 1e9, 3),
                      "frac_fp64_peak": round(term.flops / ms / 1e9 / FP64_PEAK_TF, 4),
-                     "tile_config": "128x128" if cfg == 1 else "64x64",
-                     "autotune_ms": {("128x128" if k == 1 else "64x64"): round(v, 4) for k, v in times.items()}}
+                     "tile_config": bsp.CONFIGS[cfg],
+                     "autotune_ms": {bsp.CONFIGS[k]: round(v, 4) for k, v in times.items()}}
         log(f"[blocksparse] {name}: {out[name]}")
         del A, B, Ct, term, ts
         torch.cuda.empty_cache()

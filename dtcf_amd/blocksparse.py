@@ -242,6 +242,8 @@ class BlockTerm:
 
 
 # ---------------------------------------------------------------- autotuner
+# tile configurations of dtcf_grouped_plan
+CONFIGS = {0: "64x64", 1: "128x128", 2: "64x64 packed", 3: "128x128 packed"}
 _TUNE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_grouped.json")
 _TUNED = None
 
@@ -273,9 +275,9 @@ def pick_config(term):
     b = _bucket(term)
     if b in _TUNED:
         return int(_TUNED[b])
-    # default: large C blocks amortise better over 128x128 tiles
+    # default: packed operands; large C blocks amortise better over 128x128 tiles
     r, c, _ = (int(x) for x in b.split("-"))
-    return 1 if r >= 8 and c >= 8 else 0
+    return 3 if r >= 8 and c >= 8 else 2
 
 
 def autotune(term, reps=3):
@@ -283,7 +285,7 @@ def autotune(term, reps=3):
     term's size bucket in tuned_grouped.json and return (cfg, times)."""
     global _TUNED
     times = {}
-    for cfg in (0, 1):
+    for cfg in CONFIGS:
         term.replan(cfg)
         term.run(0.0)
         torch.cuda.synchronize()

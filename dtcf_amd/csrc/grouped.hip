@@ -49,12 +49,13 @@ struct Work {
 };
 
 struct GArgs {
-    const BOp *ops;
+    const BOp *ops;          // C ops (and A / B ops of the unpacked path)
+    const BOp *pops;         // A / B ops as packed [rows][Kp] blocks
     const int32_t *pa, *pb;  // per pair: A op, B op
     const Work *work;
     int64_t nwork;
     double alpha;
-    const double *A, *B;
+    const double *A, *B;     // operand buffers (packed copies in the packed path)
     double *C;
 };
 
@@ -73,7 +74,12 @@ __device__ inline int64_t axis_off(const Axis &a, uint32_t x)
 // pass i, wave w, lane l loads (row, k0) and (row, k0+1) with
 // row = 16*g + l%16, g = (4i+w)/2, fk = (l/16 & 1) + 2*(w & 1), sp = l/32,
 // k0 = 4*fk + 2*sp; LDS slots (g*4 + 2sp)*64 + fk*16 + r and +64.
-template <int ROWS>
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// PACKED: the block was copied to a dense [rows][Kp] array (Kp a multiple of
+// BK, zero padded), so the pair (row, k0), (row, k0+1) is one aligned
+// 16-byte load with no K bounds check.
+template <int ROWS, bool PACKED>
 struct BlockLoader {
     static constexpr int NP = ROWS / 32;
     double x[NP], y[NP];
@@ -97,7 +103,8 @@ struct BlockLoader {
             coords(i, w, l, row, k0, slot);
             const int64_t r = row0 + row;
             rok[i] = r < op.rows;
-            rofs[i] = rok[i] ? op.base + axis_off(op.r, (uint32_t)r) : 0;
+            if constexpr (PACKED) rofs[i] = rok[i] ? op.base + r * op.c.s[0] : 0;
+            else rofs[i] = rok[i] ? op.base + axis_off(op.r, (uint32_t)r) : 0;
         }
     }
 
@@ -109,8 +116,14 @@ struct BlockLoader {
             int row, k0, slot;
             coords(i, w, l, row, k0, slot);
             const int64_t c0 = kb + k0, c1 = c0 + 1;
-            x[i] = (rok[i] && c0 < op.cols) ? buf[rofs[i] + axis_off(op.c, (uint32_t)c0)] : 0.0;
-            y[i] = (rok[i] && c1 < op.cols) ? buf[rofs[i] + axis_off(op.c, (uint32_t)c1)] : 0.0;
+            if constexpr (PACKED) {
+                const d2 v = rok[i] ? *reinterpret_cast<const d2 *>(buf + rofs[i] + c0) : (d2){0.0, 0.0};
+                x[i] = v.x;
+                y[i] = v.y;
+            } else {
+                x[i] = (rok[i] && c0 < op.cols) ? buf[rofs[i] + axis_off(op.c, (uint32_t)c0)] : 0.0;
+                y[i] = (rok[i] && c1 < op.cols) ? buf[rofs[i] + axis_off(op.c, (uint32_t)c1)] : 0.0;
+            }
         }
     }
 
@@ -131,7 +144,7 @@ struct GOcc {
     static constexpr int v = (BM * BN <= 64 * 64) ? 4 : 2;
 };
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, bool PACKED>
 __global__ __launch_bounds__(NT, (GOcc<BM, BN>::v)) void grouped_kernel(GArgs g)
 {
     static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
@@ -145,7 +158,7 @@ __global__ __launch_bounds__(NT, (GOcc<BM, BN>::v)) void grouped_kernel(GArgs g)
 
     for (int64_t w = blockIdx.x; w < g.nwork; w += gridDim.x) {
         const Work wk = g.work[w];
-        const BOp cop = g.ops[wk.c_op];
+        const BOp &cop = g.ops[wk.c_op];  // read in place: its axes are indexed at run time
         const int64_t m0 = (int64_t)wk.tm * BM, n0 = (int64_t)wk.tn * BN;
         d4 acc[FM][FN];
 #pragma unroll
@@ -153,49 +166,73 @@ __global__ __launch_bounds__(NT, (GOcc<BM, BN>::v)) void grouped_kernel(GArgs g)
 #pragma unroll
             for (int j = 0; j < FN; j++) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
 
-        for (int p = wk.p0; p < wk.p1; p++) {
-            const BOp aop = g.ops[g.pa[p]];
-            const BOp bop = g.ops[g.pb[p]];
-            BlockLoader<BM> la;
-            BlockLoader<BN> lb;
-            la.rows_for(aop, m0, wave, lane);
-            lb.rows_for(bop, n0, wave, lane);
-            const int64_t nk = (aop.cols + BK - 1) / BK;
-            la.load(aop, g.A, 0, wave, lane);
-            lb.load(bop, g.B, 0, wave, lane);
-            __syncthreads();  // the previous pair / work item is done with LDS
-            la.store(smem, wave, lane);
-            lb.store(smem + LDS_A, wave, lane);
-            __syncthreads();
-            for (int64_t kt = 0; kt < nk; kt++) {
-                const int cur = (int)(kt & 1);
-                const bool more = kt + 1 < nk;
-                if (more) {
-                    la.load(aop, g.A, (kt + 1) * BK, wave, lane);
-                    lb.load(bop, g.B, (kt + 1) * BK, wave, lane);
+        // The pairs of the C block form one continuous K loop: the first K
+        // step of the next pair is fetched during the last step of the
+        // current one, so there is no pipeline restart between pairs.
+        const BOp *opsAB = PACKED ? g.pops : g.ops;
+        int p = wk.p0;
+        // op indices, not copies: the descriptors are read where used, so
+        // the pair switch needs no per-lane storage
+        int ia = g.pa[p], ib = g.pb[p];
+        BlockLoader<BM, PACKED> la;
+        BlockLoader<BN, PACKED> lb;
+        la.rows_for(opsAB[ia], m0, wave, lane);
+        lb.rows_for(opsAB[ib], n0, wave, lane);
+        int64_t nk = (opsAB[ia].cols + BK - 1) / BK;  // packed: Kp / BK, no tail
+        la.load(opsAB[ia], g.A, 0, wave, lane);
+        lb.load(opsAB[ib], g.B, 0, wave, lane);
+        __syncthreads();  // the previous work item is done with LDS
+        la.store(smem, wave, lane);
+        lb.store(smem + LDS_A, wave, lane);
+        __syncthreads();
+        int64_t kt = 0;
+        int cur = 0;
+        for (;;) {
+            const bool last = kt + 1 == nk;
+            const bool more = !last || p + 1 < wk.p1;
+            if (more) {
+                if (last) {
+                    const int na = g.pa[p + 1], nb = g.pb[p + 1];
+                    la.rows_for(opsAB[na], m0, wave, lane);
+                    lb.rows_for(opsAB[nb], n0, wave, lane);
+                    la.load(opsAB[na], g.A, 0, wave, lane);
+                    lb.load(opsAB[nb], g.B, 0, wave, lane);
+                } else {
+                    la.load(opsAB[ia], g.A, (kt + 1) * BK, wave, lane);
+                    lb.load(opsAB[ib], g.B, (kt + 1) * BK, wave, lane);
                 }
-                const double *a_s = smem + cur * STAGE + gA * 256 + lane;
-                const double *b_s = smem + cur * STAGE + LDS_A + gB * 256 + lane;
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    double af[FM], bf[FN];
-#pragma unroll
-                    for (int i = 0; i < FM; i++) af[i] = a_s[i * 256 + s * 64];
-#pragma unroll
-                    for (int j = 0; j < FN; j++) bf[j] = b_s[j * 256 + s * 64];
-#pragma unroll
-                    for (int i = 0; i < FM; i++)
-#pragma unroll
-                        for (int j = 0; j < FN; j++)
-                            acc[i][j] =
-                                __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-                    if (s == 1 && more) {
-                        la.store(smem + (cur ^ 1) * STAGE, wave, lane);
-                        lb.store(smem + (cur ^ 1) * STAGE + LDS_A, wave, lane);
-                    }
-                }
-                __syncthreads();
             }
+            const double *a_s = smem + cur * STAGE + gA * 256 + lane;
+            const double *b_s = smem + cur * STAGE + LDS_A + gB * 256 + lane;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                double af[FM], bf[FN];
+#pragma unroll
+                for (int i = 0; i < FM; i++) af[i] = a_s[i * 256 + s * 64];
+#pragma unroll
+                for (int j = 0; j < FN; j++) bf[j] = b_s[j * 256 + s * 64];
+#pragma unroll
+                for (int i = 0; i < FM; i++)
+#pragma unroll
+                    for (int j = 0; j < FN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                if (s == 1 && more) {
+                    la.store(smem + (cur ^ 1) * STAGE, wave, lane);
+                    lb.store(smem + (cur ^ 1) * STAGE + LDS_A, wave, lane);
+                }
+            }
+            __syncthreads();
+            if (!more) break;
+            if (last) {
+                p++;
+                ia = g.pa[p];
+                ib = g.pb[p];
+                nk = (opsAB[ia].cols + BK - 1) / BK;
+                kt = 0;
+            } else {
+                kt++;
+            }
+            cur ^= 1;
         }
 #pragma unroll
         for (int i = 0; i < FM; i++)
@@ -233,6 +270,33 @@ int64_t axis_len(const dtcf_axis &a)
     return n;
 }
 
+// Packing: every A and B block of the term copied once into a dense
+// [rows][Kp] array (Kp = cols rounded up to BK, padding zeroed).  Work item
+// = 8 rows of one block; threads run along the packed row, so the writes are
+// coalesced and the reads are wherever the block's column digits point.
+struct PackWork {
+    int32_t op, r0;
+};
+
+__global__ __launch_bounds__(256) void pack_kernel(const BOp *ops, const BOp *pops,
+                                                   const PackWork *work, int64_t nwork,
+                                                   const double *__restrict__ src,
+                                                   double *__restrict__ dst)
+{
+    for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const PackWork pw = work[w];
+        const BOp o = ops[pw.op], q = pops[pw.op];
+        const int64_t kp = q.c.s[0];
+        const int r1 = (int)min<int64_t>(pw.r0 + 8, o.rows);
+        for (int r = pw.r0; r < r1; r++) {
+            const int64_t roff = o.base + axis_off(o.r, (uint32_t)r);
+            double *d = dst + q.base + (int64_t)r * kp;
+            for (int64_t c = threadIdx.x; c < kp; c += 256)
+                d[c] = c < o.cols ? src[roff + axis_off(o.c, (uint32_t)c)] : 0.0;
+        }
+    }
+}
+
 int64_t axis_max(const dtcf_axis &a)
 {
     int64_t m = 0;
@@ -247,7 +311,11 @@ using namespace dtcf;
 
 struct dtcf_grouped_s {
     int cfg = 0;
-    BOp *ops = nullptr;
+    bool packed = false;
+    BOp *ops = nullptr, *pops = nullptr;
+    double *packA = nullptr, *packB = nullptr;
+    PackWork *wA = nullptr, *wB = nullptr;
+    int64_t nwA = 0, nwB = 0;
     int32_t *pa = nullptr, *pb = nullptr;
     Work *work = nullptr;
     int64_t nwork = 0, nops = 0, npairs = 0;
@@ -259,7 +327,8 @@ struct dtcf_grouped_s {
 extern "C" int dtcf_grouped_destroy(dtcf_grouped g)
 {
     if (!g) return DTCF_OK;
-    for (void *p : {(void *)g->ops, (void *)g->pa, (void *)g->pb, (void *)g->work})
+    for (void *p : {(void *)g->ops, (void *)g->pa, (void *)g->pb, (void *)g->work, (void *)g->pops,
+                    (void *)g->packA, (void *)g->packB, (void *)g->wA, (void *)g->wB})
         if (p) (void)hipFree(p);
     delete g;
     return DTCF_OK;
@@ -273,8 +342,9 @@ extern "C" int dtcf_grouped_plan(const dtcf_block_op *ops, int64_t nops,
     *out = nullptr;
     DTCF_REQUIRE(nops >= 0 && npairs >= 0 && (nops == 0 || ops) && (npairs == 0 || pairs),
                  "grouped: bad op or pair list");
-    DTCF_REQUIRE(cfg >= 0 && cfg < 2, "grouped: tile configuration must be 0 (64x64) or 1 (128x128)");
-    const int BM = cfg == 0 ? 64 : 128;
+    DTCF_REQUIRE(cfg >= 0 && cfg < 4,
+                 "grouped: configuration 0 (64x64), 1 (128x128), 2 (packed 64x64) or 3 (packed 128x128)");
+    const int BM = (cfg & 1) == 0 ? 64 : 128;
     std::vector<BOp> hops(nops);
     for (int64_t i = 0; i < nops; i++) {
         const dtcf_block_op &o = ops[i];
@@ -349,10 +419,47 @@ extern "C" int dtcf_grouped_plan(const dtcf_block_op *ops, int64_t nops,
     g->nwork = (int64_t)sorted.size();
     g->nops = nops;
     g->npairs = npairs;
+    // Packed configurations: a dense [rows][Kp] slot per A op and per B op.
+    g->packed = cfg >= 2;
+    std::vector<BOp> hpops;
+    std::vector<PackWork> hwA, hwB;
+    int64_t lenA = 0, lenB = 0;
+    if (g->packed) {
+        hpops = hops;
+        std::vector<char> isA(nops, 0), isB(nops, 0);
+        for (int64_t j = 0; j < npairs; j++) {
+            isA[hpa[j]] = 1;
+            isB[hpb[j]] = 1;
+        }
+        for (int64_t i = 0; i < nops; i++) {
+            if (!isA[i] && !isB[i]) continue;
+            DTCF_REQUIRE(!(isA[i] && isB[i]), "grouped: an op used both as A and as B");
+            const int64_t kp = (hops[i].cols + BK - 1) / BK * BK;
+            int64_t &len = isA[i] ? lenA : lenB;
+            BOp &q = hpops[i];
+            q.base = len;
+            q.r = Axis{};
+            q.r.nd = 1;
+            q.r.e[0] = (uint32_t)hops[i].rows;
+            q.r.d[0] = FastDiv((uint32_t)hops[i].rows);
+            q.r.s[0] = kp;
+            q.c = Axis{};
+            q.c.nd = 1;
+            q.c.e[0] = (uint32_t)kp;
+            q.c.d[0] = FastDiv((uint32_t)kp);
+            q.c.s[0] = kp;  // the row pitch, read by the loader and the pack kernel
+            q.cols = kp;
+            len += hops[i].rows * kp;
+            for (int64_t r = 0; r < hops[i].rows; r += 8)
+                (isA[i] ? hwA : hwB).push_back(PackWork{(int32_t)i, (int32_t)r});
+        }
+        g->nwA = (int64_t)hwA.size();
+        g->nwB = (int64_t)hwB.size();
+    }
     int dev = 0, cus = 1;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int per_cu = cfg == 0 ? 4 : 2;
+    const int per_cu = (cfg & 1) == 0 ? 4 : 2;
     g->grid = (int)std::min<int64_t>(std::max<int64_t>(g->nwork, 1), (int64_t)cus * per_cu);
     if (e == hipSuccess && nops) e = hipMalloc(&g->ops, nops * sizeof(BOp));
     if (e == hipSuccess && npairs) e = hipMalloc(&g->pa, npairs * sizeof(int32_t));
@@ -363,6 +470,19 @@ extern "C" int dtcf_grouped_plan(const dtcf_block_op *ops, int64_t nops,
     if (e == hipSuccess && npairs) e = hipMemcpy(g->pb, hpb.data(), npairs * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess && g->nwork)
         e = hipMemcpy(g->work, sorted.data(), g->nwork * sizeof(Work), hipMemcpyHostToDevice);
+    if (g->packed) {
+        if (e == hipSuccess && nops) e = hipMalloc(&g->pops, nops * sizeof(BOp));
+        if (e == hipSuccess && nops)
+            e = hipMemcpy(g->pops, hpops.data(), nops * sizeof(BOp), hipMemcpyHostToDevice);
+        if (e == hipSuccess && lenA) e = hipMalloc(&g->packA, lenA * sizeof(double));
+        if (e == hipSuccess && lenB) e = hipMalloc(&g->packB, lenB * sizeof(double));
+        if (e == hipSuccess && g->nwA) e = hipMalloc(&g->wA, g->nwA * sizeof(PackWork));
+        if (e == hipSuccess && g->nwB) e = hipMalloc(&g->wB, g->nwB * sizeof(PackWork));
+        if (e == hipSuccess && g->nwA)
+            e = hipMemcpy(g->wA, hwA.data(), g->nwA * sizeof(PackWork), hipMemcpyHostToDevice);
+        if (e == hipSuccess && g->nwB)
+            e = hipMemcpy(g->wB, hwB.data(), g->nwB * sizeof(PackWork), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         dtcf_grouped_destroy(g);
         return hip_fail(e, "grouped: plan upload");
@@ -390,12 +510,27 @@ extern "C" int dtcf_grouped_run(dtcf_grouped g, double alpha, const double *A, c
     DTCF_REQUIRE(g, "grouped: null plan");
     if (g->nwork == 0) return DTCF_OK;
     DTCF_REQUIRE(A && B && C, "grouped: null buffer");
-    GArgs a{g->ops, g->pa, g->pb, g->work, g->nwork, alpha, A, B, C};
+    GArgs a{g->ops, g->pops, g->pa, g->pb, g->work, g->nwork, alpha, A, B, C};
     hipStream_t s = as_stream(stream);
-    if (g->cfg == 0)
-        hipLaunchKernelGGL((grouped_kernel<64, 64, 32, 32>), dim3(g->grid), dim3(NT), 0, s, a);
-    else
-        hipLaunchKernelGGL((grouped_kernel<128, 128, 64, 64>), dim3(g->grid), dim3(NT), 0, s, a);
+    if (g->packed) {
+        // the packed copies belong to the plan: runs of one plan on several
+        // streams at once would race on them
+        if (g->nwA)
+            hipLaunchKernelGGL(pack_kernel, dim3((unsigned)std::min<int64_t>(g->nwA, 65536)), dim3(256), 0,
+                               s, g->ops, g->pops, g->wA, g->nwA, A, g->packA);
+        if (g->nwB)
+            hipLaunchKernelGGL(pack_kernel, dim3((unsigned)std::min<int64_t>(g->nwB, 65536)), dim3(256), 0,
+                               s, g->ops, g->pops, g->wB, g->nwB, B, g->packB);
+        DTCF_HIP(hipGetLastError());
+        a.A = g->packA;
+        a.B = g->packB;
+    }
+    switch (g->cfg) {
+    case 0: hipLaunchKernelGGL((grouped_kernel<64, 64, 32, 32, false>), dim3(g->grid), dim3(NT), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((grouped_kernel<128, 128, 64, 64, false>), dim3(g->grid), dim3(NT), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((grouped_kernel<64, 64, 32, 32, true>), dim3(g->grid), dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((grouped_kernel<128, 128, 64, 64, true>), dim3(g->grid), dim3(NT), 0, s, a); break;
+    }
     DTCF_HIP(hipGetLastError());
     return DTCF_OK;
 }

@@ -293,7 +293,12 @@ typedef struct {
 
 typedef struct dtcf_grouped_s *dtcf_grouped;
 
-/* cfg: 0 = 64x64 C tiles per work item, 1 = 128x128 (autotuned per term). */
+/* cfg: 0 = 64x64 C tiles per work item, 1 = 128x128, operands read in place
+ * through their descriptors; 2 = 64x64, 3 = 128x128 with every A and B block
+ * first copied (one pack launch per operand, inside dtcf_grouped_run) to a
+ * dense zero-padded [rows][K] slot owned by the plan, so the GEMM reads it
+ * with 16-byte loads.  The packed copies make a plan unsafe to run on two
+ * streams at once.  Autotuned per term. */
 int dtcf_grouped_plan(const dtcf_block_op *ops, int64_t nops, const dtcf_block_pair *pairs,
                       int64_t npairs, int cfg, dtcf_grouped *out);
 /* Algorithmic flops (sum of 2*m*n*k over pairs), work items, and the buffer
