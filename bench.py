@@ -151,38 +151,12 @@ def leg_precision(log):
 def leg_blocksparse(dev, log, steps=5):
     """Block-sparse D2h terms (o=64, v=256, 8 irreps): one grouped persistent
     launch per term over its symmetry-allowed block pairs."""
-    import torch
-    from dtcf_amd import blocksparse as bsp, mixed
-    sp = bsp.Space.seeded(o=64, v=256, seed=5)
-    terms = {"ring ijab+=ikac*kjcb": (("oovv", "ijab"), ("oovv", "ikac"), ("oovv", "kjcb")),
-             "ladder abij+=abcd*cdij": (("vvoo", "abij"), ("vvvv", "abcd"), ("vvoo", "cdij"))}
-    out = {}
-    for name, ((ck, ci), (ak, ai), (bk, bi)) in terms.items():
-        ts = []
-        for seed, kinds in ((1, ak), (2, bk), (3, ck)):
-            t = bsp.BSTensor(sp, kinds, dev)
-            t.data.copy_(mixed.fill_normal(t.size, seed, 0.05))
-            ts.append(t)
-        A, B, Ct = ts
-        term = bsp.BlockTerm(Ct, ci, A, ai, B, bi)
-        cfg, times = bsp.autotune(term)
-        term.run(1.0)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            term.run(1.0)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
-        out[name] = {"block_pairs": term.npairs, "flops_performed": term.flops, "ms": round(ms, 4),
-                     "tflops": round(term.flops / ms / 1e9, 3),
-                     "frac_fp64_peak": round(term.flops / ms / 1e9 / FP64_PEAK_TF, 4),
-                     "tile_config": bsp.CONFIGS[cfg],
-                     "autotune_ms": {bsp.CONFIGS[k]: round(v, 4) for k, v in times.items()}}
-        log(f"[blocksparse] {name}: {out[name]}")
-        del A, B, Ct, term, ts
-        torch.cuda.empty_cache()
+    from dtcf_amd import blocksparse as bsp
+    out = bsp.bench_terms(dev, steps=steps)
+    for k, v in out.items():
+        log(f"[blocksparse] {k}: {v}")
+    for v in out.values():
+        v["frac_fp64_peak"] = round(v["tflops"] / FP64_PEAK_TF, 4)
     return out
 
 

@@ -307,3 +307,52 @@ def autotune(term, reps=3):
 def save_tuning(path=None):
     with open(path or _TUNE_PATH, "w") as f:
         json.dump(_TUNED or {}, f, indent=1, sort_keys=True)
+
+
+# ---------------------------------------------------------------- benchmark
+BENCH_TERMS = {"ring ijab+=ikac*kjcb": (("oovv", "ijab"), ("oovv", "ikac"), ("oovv", "kjcb")),
+               "ladder abij+=abcd*cdij": (("vvoo", "abij"), ("vvvv", "abcd"), ("vvoo", "cdij"))}
+
+
+def bench_terms(dev="cuda", o=64, v=256, seed=5, steps=5):
+    """The D2h ring and ladder terms at o, v: autotune each, then time
+    `steps` runs (HIP events).  Returns per-term block pairs, performed flops,
+    ms per run (pack launches included) and TFLOP/s."""
+    from . import mixed
+    sp = Space.seeded(o=o, v=v, seed=seed)
+    out = {}
+    for name, ((ck, ci), (ak, ai), (bk, bi)) in BENCH_TERMS.items():
+        ts = []
+        for sd, kinds in ((1, ak), (2, bk), (3, ck)):
+            t = BSTensor(sp, kinds, dev)
+            t.data.copy_(mixed.fill_normal(t.size, sd, 0.05))
+            ts.append(t)
+        A, B, Ct = ts
+        term = BlockTerm(Ct, ci, A, ai, B, bi)
+        cfg, times = autotune(term)
+        term.run(1.0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            term.run(1.0)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out[name] = {"block_pairs": term.npairs, "flops_performed": term.flops, "ms": round(ms, 4),
+                     "tflops": round(term.flops / ms / 1e9, 3), "tile_config": CONFIGS[cfg],
+                     "autotune_ms": {CONFIGS[k]: round(x, 4) for k, x in times.items()}}
+        del A, B, Ct, term, ts
+        torch.cuda.empty_cache()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="D2h block-sparse ring and ladder terms, grouped launch")
+    ap.add_argument("--o", type=int, default=64)
+    ap.add_argument("--v", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=5)
+    args = ap.parse_args()
+    check(lib().dtcf_init(0))
+    print(json.dumps(bench_terms("cuda", args.o, args.v, steps=args.steps), indent=1), flush=True)
