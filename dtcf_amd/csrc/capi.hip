@@ -246,12 +246,12 @@ static void fill_op(OpArgs &o, const dtcf_view_s *v)
     o.ncb = v->ncb;
 }
 
-extern "C" int dtcf_gemm_view_f64(int64_t m, int64_t n, int64_t k, double alpha,
-                                  const double *A, dtcf_view av, const double *B, dtcf_view bv,
-                                  double *C, dtcf_view cv, int64_t ldc, void *stream)
+static int gemm_view_impl(int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                          dtcf_view av, const double *B, dtcf_view bv, double *C, dtcf_view cv,
+                          int64_t ldc, void *stream, int overwrite)
 {
     DTCF_REQUIRE(m >= 0 && n >= 0 && k >= 0, "gemm_view: negative size");
-    if (m == 0 || n == 0 || k == 0 || alpha == 0.0) return DTCF_OK;
+    if (m == 0 || n == 0 || k == 0 || (alpha == 0.0 && !overwrite)) return DTCF_OK;
     DTCF_REQUIRE(A && B && C && av && bv, "gemm_view: null operand or view");
     DTCF_REQUIRE(av->rows == m && av->cols == k, "gemm_view: A view is not m x k");
     DTCF_REQUIRE(bv->rows == n && bv->cols == k, "gemm_view: B view is not n x k");
@@ -260,6 +260,7 @@ extern "C" int dtcf_gemm_view_f64(int64_t m, int64_t n, int64_t k, double alpha,
     g.n = n;
     g.k = k;
     g.alpha = alpha;
+    g.overwrite = overwrite;
     g.A = A;
     g.B = B;
     g.C = C;
@@ -285,6 +286,20 @@ extern "C" int dtcf_gemm_view_f64(int64_t m, int64_t n, int64_t k, double alpha,
     g.c.ld = ldc;
     g.cview = 0;
     return gemm_f64_launch(g, pick_config(m, n, k), as_stream(stream));
+}
+
+extern "C" int dtcf_gemm_view_f64(int64_t m, int64_t n, int64_t k, double alpha,
+                                  const double *A, dtcf_view av, const double *B, dtcf_view bv,
+                                  double *C, dtcf_view cv, int64_t ldc, void *stream)
+{
+    return gemm_view_impl(m, n, k, alpha, A, av, B, bv, C, cv, ldc, stream, 0);
+}
+
+int dtcf::gemm_view_overwrite(int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                              dtcf_view av, const double *B, dtcf_view bv, double *C, dtcf_view cv,
+                              int64_t ldc, hipStream_t stream)
+{
+    return gemm_view_impl(m, n, k, alpha, A, av, B, bv, C, cv, ldc, stream, 1);
 }
 
 // ---------------------------------------------------------------------------

@@ -359,7 +359,8 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
         return;
     }
 
-    // Epilogue: C += alpha * acc  (beta = 1, as every reference call site).
+    // Epilogue: C += alpha * acc  (beta = 1, as every reference call site),
+    // or C = alpha * acc when the caller asked for an overwrite.
     const double alpha = g.alpha;
 #pragma unroll
     for (int i = 0; i < FM; i++)
@@ -374,7 +375,8 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
                 int64_t addr;
                 if constexpr (CVIEW) addr = op_addr<kVK>(g.c, row, col);
                 else addr = row * g.c.ld + col;
-                g.C[addr] += alpha * acc[i][j][r];
+                if (g.overwrite) g.C[addr] = alpha * acc[i][j][r];
+                else g.C[addr] += alpha * acc[i][j][r];
             }
         }
 }
@@ -402,7 +404,8 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(GemmArgs g)
         int64_t addr;
         if constexpr (CVIEW) addr = op_addr<kVK>(g.c, row, col);
         else addr = row * g.c.ld + col;
-        g.C[addr] += g.alpha * sum;
+        if (g.overwrite) g.C[addr] = g.alpha * sum;
+        else g.C[addr] += g.alpha * sum;
     }
 }
 

@@ -43,6 +43,9 @@ struct GemmArgs {
     uint32_t t_full, sk;
     int64_t kchunk;
     double *ws;
+    // 0: C += alpha * A * B (every reference call site); 1: C = alpha * A * B
+    // (C is neither cleared first nor read)
+    int32_t overwrite;
 };
 
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
@@ -50,5 +53,10 @@ int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
 // Allocates the tail-split workspace of stream `s` at the largest size any
 // GEMM on it can need, so later launches on `s` never allocate.
 int gemm_reserve_workspace(hipStream_t s);
+
+// dtcf_gemm_view_f64 with C = alpha * A * B instead of C += (capi.hip).
+int gemm_view_overwrite(int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                        dtcf_view av, const double *B, dtcf_view bv, double *C, dtcf_view cv,
+                        int64_t ldc, hipStream_t stream);
 
 }  // namespace dtcf

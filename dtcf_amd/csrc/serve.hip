@@ -490,11 +490,10 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
         DTCF_HIP(hipGetLastError());
         return DTCF_OK;
     }
-    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(C.elems)), dim3(256), 0, stream, c, C.elems);
-    DTCF_HIP(hipGetLastError());
     const double *A = arr_ptr(p, p->arrs[s.a], r), *B = arr_ptr(p, p->arrs[s.b], r);
     if (s.swap) std::swap(A, B);
     if (p->prec == 0) {
+        // FP64: the GEMM writes C = A * B directly (no clearing pass)
         const double *op[2] = {A, B};
         for (int q = 0; q < 2; q++) {
             if (!s.pack[q]) continue;
@@ -504,9 +503,12 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
             if (st != DTCF_OK) return st;
             op[q] = dst;
         }
-        return dtcf_gemm_view_f64(s.m, s.n, s.k, s.alpha, op[0], s.av, op[1], s.bv, c, s.cv, s.ldc,
-                                  stream);
+        return gemm_view_overwrite(s.m, s.n, s.k, s.alpha, op[0], s.av, op[1], s.bv, c, s.cv, s.ldc,
+                                   stream);
     }
+    // mixed precision accumulates its products into a cleared C
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(C.elems)), dim3(256), 0, stream, c, C.elems);
+    DTCF_HIP(hipGetLastError());
     char *scr = r->base + s.scratch_off;
     int32_t *ea = reinterpret_cast<int32_t *>(scr + s.ea_off);
     int32_t *eb = reinterpret_cast<int32_t *>(scr + s.eb_off);
