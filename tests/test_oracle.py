@@ -293,3 +293,44 @@ def test_dltc_task_mul_restatement_vs_einsum():
     X = np.zeros_like(Y)
     oracle.dltc_task_sum([12, 13, 0, 1], X, [13, 12, 1, 0], Y, 1.0, wbO, wbV)
     assert np.array_equal(X, np.transpose(Y, (1, 0, 3, 2)))
+
+
+def test_bs_contract_vs_dense_einsum():
+    """Block-sparse restatement against the dense einsum of the same tensors
+    with the symmetry-forbidden blocks zero: a product of two allowed blocks
+    only lands in an allowed C block, so the two agree on every C block."""
+    import itertools
+    from dtcf_amd import blocksparse as bsp
+    sp = bsp.Space([1, 2, 1, 1, 2, 1, 1, 1], [2, 3, 2, 2, 1, 2, 3, 2])
+    rng = np.random.default_rng(4)
+
+    def make(kinds):
+        t = bsp.BSTensor(sp, kinds, device=None)
+        buf = rng.standard_normal(t.size)
+        starts = {k: np.concatenate([[0], np.cumsum(sp.counts[k])]) for k in "ov"}
+        dense = np.zeros([sum(sp.counts[k]) for k in kinds])
+        for g in t.blocks:
+            sl = tuple(slice(starts[k][gi], starts[k][gi] + sp.counts[k][gi]) for k, gi in zip(kinds, g))
+            off = t.offset[g]
+            dense[sl] = buf[off:off + int(np.prod(t.shape[g]))].reshape(t.shape[g])
+        meta = {g: (t.offset[g], t.shape[g]) for g in t.blocks}
+        return t, buf, dense, meta, starts
+
+    for (ck, ci), (ak, ai), (bk, bi) in ((("oovv", "ijab"), ("oovv", "ikac"), ("oovv", "kjcb")),
+                                          (("vvoo", "abij"), ("vvvv", "abcd"), ("vvoo", "cdij"))):
+        At, Ab, Ad, Am, _ = make(ak)
+        Bt, Bb, Bd, Bm, _ = make(bk)
+        Ct, Cb, Cd, Cm, starts = make(ck)
+        got = oracle.bs_contract(Cm, Cb.copy(), ci, Am, Ab, ai, Bm, Bb, bi, 0.7)
+        dense = Cd + 0.7 * np.einsum(f"{ai},{bi}->{ci}", Ad, Bd)
+        for g in Ct.blocks:
+            sl = tuple(slice(starts[k][gi], starts[k][gi] + sp.counts[k][gi]) for k, gi in zip(ck, g))
+            off = Ct.offset[g]
+            blk = got[off:off + int(np.prod(Ct.shape[g]))].reshape(Ct.shape[g])
+            np.testing.assert_allclose(blk, dense[sl], rtol=1e-13, atol=1e-13)
+        # forbidden C blocks of the dense result stay zero
+        for g in itertools.product(range(8), repeat=4):
+            if g in Ct.offset:
+                continue
+            sl = tuple(slice(starts[k][gi], starts[k][gi] + sp.counts[k][gi]) for k, gi in zip(ck, g))
+            assert np.abs(dense[sl] - Cd[sl]).max(initial=0.0) < 1e-13
