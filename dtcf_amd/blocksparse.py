@@ -52,7 +52,14 @@ def _declare():
                                     C.POINTER(C.c_int64)]
     L.dtcf_grouped_run.argtypes = [vp, C.c_double, vp, vp, vp, vp]
     L.dtcf_grouped_destroy.argtypes = [vp]
-    for f in ("dtcf_grouped_plan", "dtcf_grouped_info", "dtcf_grouped_run", "dtcf_grouped_destroy"):
+    L.dtcf_blockmove_plan.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(vp)]
+    L.dtcf_blockmove_info.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.dtcf_blockmove_gather_f64.argtypes = [vp, vp, vp, vp]
+    L.dtcf_blockmove_scatter_f64.argtypes = [vp, C.c_double, vp, vp, vp]
+    L.dtcf_blockmove_destroy.argtypes = [vp]
+    for f in ("dtcf_grouped_plan", "dtcf_grouped_info", "dtcf_grouped_run", "dtcf_grouped_destroy",
+              "dtcf_blockmove_plan", "dtcf_blockmove_info", "dtcf_blockmove_gather_f64",
+              "dtcf_blockmove_scatter_f64", "dtcf_blockmove_destroy"):
         getattr(L, f).restype = C.c_int
 
 
@@ -146,6 +153,168 @@ def _strides(shape):
     return st
 
 
+def block_op(t: BSTensor, idx, g, row_names, col_names):
+    """Descriptor of block g of t (index names idx) as a rows x cols matrix:
+    rows over row_names, columns over col_names (each in the given order)."""
+    shp = t.shape[g]
+    st = _strides(shp)
+    pos = {x: i for i, x in enumerate(idx)}
+    op = BlockOp()
+    op.base = t.offset[g]
+    op.rows = _axis([shp[pos[x]] for x in row_names], [st[pos[x]] for x in row_names])
+    op.cols = _axis([shp[pos[x]] for x in col_names], [st[pos[x]] for x in col_names])
+    return op
+
+
+class BlockMove:
+    """dtcf_blockmove plan: blocks of one tensor <-> placed regions of a
+    dense buffer."""
+
+    def __init__(self, ops, bases, pitches):
+        n = len(ops)
+        self.ops = (BlockOp * max(1, n))(*ops)
+        self.bases = (C.c_int64 * max(1, n))(*bases)
+        self.pitches = (C.c_int64 * max(1, n))(*pitches)
+        h = C.c_void_p()
+        check(lib().dtcf_blockmove_plan(C.cast(self.ops, C.c_void_p), C.cast(self.bases, C.c_void_p),
+                                        C.cast(self.pitches, C.c_void_p), n, C.byref(h)))
+        self.h = h
+        sp = C.c_int64()
+        check(lib().dtcf_blockmove_info(h, C.byref(sp)))
+        self.span = sp.value
+
+    def gather(self, sparse, dense):
+        check(lib().dtcf_blockmove_gather_f64(self.h, C.c_void_p(sparse.data_ptr()),
+                                              C.c_void_p(dense.data_ptr()), _s()))
+
+    def scatter(self, alpha, dense, sparse):
+        check(lib().dtcf_blockmove_scatter_f64(self.h, float(alpha), C.c_void_p(dense.data_ptr()),
+                                               C.c_void_p(sparse.data_ptr()), _s()))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().dtcf_blockmove_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _s():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _xor(g):
+    x = 0
+    for v in g:
+        x ^= v
+    return x
+
+
+class IrrepDense:
+    """A block-sparse term as one dense GEMM per irrep.
+
+    With h the product (XOR) of the irreps of C's row indices, an allowed C
+    block has column irreps of product h too, and an A (B) block feeding it
+    has contracted irreps of product h.  For each h every (row block, column
+    block, contracted block) combination occurs, so the term is
+        C_h[M_h x N_h] += A_h[M_h x K_h] * B_h[N_h x K_h]^T
+    over the blocks of class h laid side by side.  A and B are gathered into
+    the A_h / B_h matrices (one launch each), the GEMMs write C_h (FP64 MFMA,
+    overwriting), and C_h is added back into C's blocks (one launch).  The
+    flops are exactly those of the block pairs -- nothing is padded."""
+
+    def __init__(self, term):
+        t = term
+        sp, kind = t.C.space, t.kind
+        a_ext, b_ext, contr = t.a_ext, t.b_ext, t.contr
+
+        def tuples(names, h):
+            return [g for g in itertools.product(range(NIRREP), repeat=len(names)) if _xor(g) == h]
+
+        def size(names, g):
+            return int(np.prod([sp.counts[kind[x]][gi] for x, gi in zip(names, g)]))
+
+        def assemble(idx, names1, g1, names2, g2):
+            d = dict(zip(names1, g1))
+            d.update(zip(names2, g2))
+            return tuple(d[x] for x in idx)
+
+        aops, abase, apitch = [], [], []
+        bops, bbase, bpitch = [], [], []
+        cops, cbase, cpitch = [], [], []
+        self.gemms = []  # (m, n, k, a_off, b_off, c_off)
+        la = lb = lc = 0
+        for h in range(NIRREP):
+            R, Cc, Kc = tuples(a_ext, h), tuples(b_ext, h), tuples(contr, h)
+            rs = [size(a_ext, g) for g in R]
+            cs = [size(b_ext, g) for g in Cc]
+            ks = [size(contr, g) for g in Kc]
+            M, N, K = sum(rs), sum(cs), sum(ks)
+            if M == 0 or N == 0 or K == 0:
+                continue
+            roff = np.concatenate([[0], np.cumsum(rs)])
+            coff = np.concatenate([[0], np.cumsum(cs)])
+            koff = np.concatenate([[0], np.cumsum(ks)])
+            for i, gr in enumerate(R):
+                for k, gk in enumerate(Kc):
+                    ga = assemble(t.a_idx, a_ext, gr, contr, gk)
+                    aops.append(block_op(t.A, t.a_idx, ga, a_ext, contr))
+                    abase.append(la + int(roff[i]) * K + int(koff[k]))
+                    apitch.append(K)
+            for j, gc in enumerate(Cc):
+                for k, gk in enumerate(Kc):
+                    gb = assemble(t.b_idx, b_ext, gc, contr, gk)
+                    bops.append(block_op(t.B, t.b_idx, gb, b_ext, contr))
+                    bbase.append(lb + int(coff[j]) * K + int(koff[k]))
+                    bpitch.append(K)
+            for i, gr in enumerate(R):
+                for j, gc in enumerate(Cc):
+                    gcb = assemble(t.c_idx, a_ext, gr, b_ext, gc)
+                    cops.append(block_op(t.C, t.c_idx, gcb, a_ext, b_ext))
+                    cbase.append(lc + int(roff[i]) * N + int(coff[j]))
+                    cpitch.append(N)
+            self.gemms.append((M, N, K, la, lb, lc))
+            la += M * K
+            lb += N * K
+            lc += M * N
+        self.mA, self.mB, self.mC = (BlockMove(aops, abase, apitch), BlockMove(bops, bbase, bpitch),
+                                     BlockMove(cops, cbase, cpitch))
+        dev = t.C.data.device
+        self.dA = torch.empty(max(1, la), dtype=torch.float64, device=dev)
+        self.dB = torch.empty(max(1, lb), dtype=torch.float64, device=dev)
+        self.dC = torch.empty(max(1, lc), dtype=torch.float64, device=dev)
+        self.flops = sum(2.0 * m * n * k for m, n, k, *_ in self.gemms)
+
+    NSTREAMS = 4
+
+    def run(self, alpha, A, B, Cb):
+        """Gathers, the per-irrep GEMMs spread over NSTREAMS side streams
+        (default priority; each GEMM alone fills only part of the GPU), then
+        the scatter, ordered by events on the caller's stream."""
+        from . import ops
+        main = torch.cuda.current_stream()
+        if not hasattr(self, "_streams"):
+            self._streams = [torch.cuda.Stream(device=main.device) for _ in range(self.NSTREAMS)]
+        self.mA.gather(A, self.dA)
+        self.mB.gather(B, self.dB)
+        self.dC.zero_()
+        ready = torch.cuda.Event()
+        ready.record(main)
+        done = []
+        for i, (m, n, k, ao, bo, co) in enumerate(self.gemms):
+            st = self._streams[i % self.NSTREAMS]
+            st.wait_event(ready)
+            with torch.cuda.stream(st):
+                ops.gemm(0, 1, m, n, k, 1.0, self.dA[ao:ao + m * k], k, self.dB[bo:bo + n * k], k,
+                         self.dC[co:co + m * n], n)
+        for st in self._streams:
+            ev = torch.cuda.Event()
+            ev.record(st)
+            main.wait_event(ev)
+        self.mC.scatter(alpha, self.dC, Cb)
+
+
 class BlockTerm:
     """C[c_idx] += alpha * A[a_idx] * B[b_idx] over allowed blocks."""
 
@@ -184,6 +353,7 @@ class BlockTerm:
             return opid[key]
 
         self.flops = 0.0
+        self.pair_flops = 0.0
         for gc in Ct.blocks:
             irr = dict(zip(c_idx, gc))
             cop = op_for(Ct, "C", c_idx, gc, a_ext, b_ext)
@@ -195,6 +365,8 @@ class BlockTerm:
                     continue
                 aop = op_for(At, "A", a_idx, ga, a_ext, contr)
                 bop = op_for(Bt, "B", b_idx, gb, b_ext, contr)
+                ext = {**dict(zip(a_idx, At.shape[ga])), **dict(zip(b_idx, Bt.shape[gb]))}
+                self.pair_flops += 2.0 * float(np.prod([ext[x] for x in ext]))
                 p = BlockPair()
                 p.c_op, p.a_op, p.b_op = cop, aop, bop
                 pairs.append(p)
@@ -202,10 +374,20 @@ class BlockTerm:
         self.pairs = (BlockPair * len(pairs))(*pairs)
         self.npairs = len(pairs)
         self.nops = len(ops)
+        self.h = None
+        self._dense = None
         self.cfg = pick_config(self) if cfg is None else cfg
         self._plan()
 
     def _plan(self):
+        if self.cfg == 4:
+            self.h = None
+            self._dense = IrrepDense(self)
+            self.nwork = len(self._dense.gemms)
+            assert abs(self._dense.flops - self.pair_flops) <= 1e-6 * max(1.0, self.pair_flops)
+            self.flops = self._dense.flops
+            return
+        self._dense = None
         h = C.c_void_p()
         check(lib().dtcf_grouped_plan(C.cast(self.ops, C.c_void_p), self.nops,
                                       C.cast(self.pairs, C.c_void_p), self.npairs, int(self.cfg),
@@ -220,7 +402,10 @@ class BlockTerm:
         assert spans[0] <= self.A.size and spans[1] <= self.B.size and spans[2] <= self.C.size
 
     def replan(self, cfg):
-        lib().dtcf_grouped_destroy(self.h)
+        if self.h:
+            lib().dtcf_grouped_destroy(self.h)
+            self.h = None
+        self._dense = None
         self.cfg = cfg
         self._plan()
 
@@ -228,6 +413,9 @@ class BlockTerm:
         A = self.A.data if A is None else A
         B = self.B.data if B is None else B
         Cb = self.C.data if Cb is None else Cb
+        if self.cfg == 4:
+            self._dense.run(alpha, A, B, Cb)
+            return
         check(lib().dtcf_grouped_run(self.h, float(alpha), C.c_void_p(A.data_ptr()),
                                      C.c_void_p(B.data_ptr()), C.c_void_p(Cb.data_ptr()),
                                      C.c_void_p(torch.cuda.current_stream().cuda_stream)))
@@ -243,7 +431,8 @@ class BlockTerm:
 
 # ---------------------------------------------------------------- autotuner
 # tile configurations of dtcf_grouped_plan
-CONFIGS = {0: "64x64", 1: "128x128", 2: "64x64 packed", 3: "128x128 packed"}
+CONFIGS = {0: "64x64", 1: "128x128", 2: "64x64 packed", 3: "128x128 packed",
+           4: "dense GEMM per irrep"}
 _TUNE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_grouped.json")
 _TUNED = None
 

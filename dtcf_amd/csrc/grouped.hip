@@ -297,6 +297,38 @@ __global__ __launch_bounds__(256) void pack_kernel(const BOp *ops, const BOp *po
     }
 }
 
+// Block moves between block-sparse storage and dense matrices: move m takes
+// block ops[m] (read through its descriptors) to dense[place.base + r * pitch
+// + c] (gather), or adds alpha times that region into the block (scatter;
+// blocks are disjoint, so no two moves write one element).  Work item = 8
+// rows of one move; threads run along the row.
+struct Place {
+    int64_t base, pitch;
+};
+
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void blockmove_kernel(const BOp *ops, const Place *places,
+                                                        const PackWork *work, int64_t nwork,
+                                                        double alpha, const double *__restrict__ in,
+                                                        double *__restrict__ out)
+{
+    for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const PackWork pw = work[w];
+        const BOp &o = ops[pw.op];
+        const Place pl = places[pw.op];
+        const int r1 = (int)min<int64_t>(pw.r0 + 8, o.rows);
+        for (int r = pw.r0; r < r1; r++) {
+            const int64_t roff = o.base + axis_off(o.r, (uint32_t)r);
+            const int64_t doff = pl.base + (int64_t)r * pl.pitch;
+            for (int64_t c = threadIdx.x; c < o.cols; c += 256) {
+                const int64_t so = roff + axis_off(o.c, (uint32_t)c);
+                if constexpr (SCATTER) out[so] += alpha * in[doff + c];
+                else out[doff + c] = in[so];
+            }
+        }
+    }
+}
+
 int64_t axis_max(const dtcf_axis &a)
 {
     int64_t m = 0;
@@ -531,6 +563,104 @@ extern "C" int dtcf_grouped_run(dtcf_grouped g, double alpha, const double *A, c
     case 2: hipLaunchKernelGGL((grouped_kernel<64, 64, 32, 32, true>), dim3(g->grid), dim3(NT), 0, s, a); break;
     default: hipLaunchKernelGGL((grouped_kernel<128, 128, 64, 64, true>), dim3(g->grid), dim3(NT), 0, s, a); break;
     }
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+// ------------------------------------------------------------- block moves
+struct dtcf_blockmove_s {
+    BOp *ops = nullptr;
+    Place *places = nullptr;
+    PackWork *work = nullptr;
+    int64_t n = 0, nwork = 0, dense_span = 0;
+};
+
+extern "C" int dtcf_blockmove_destroy(dtcf_blockmove b)
+{
+    if (!b) return DTCF_OK;
+    for (void *p : {(void *)b->ops, (void *)b->places, (void *)b->work})
+        if (p) (void)hipFree(p);
+    delete b;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_blockmove_plan(const dtcf_block_op *ops, const int64_t *dense_base,
+                                   const int64_t *dense_pitch, int64_t n, dtcf_blockmove *out)
+{
+    DTCF_REQUIRE(out, "blockmove: null output");
+    *out = nullptr;
+    DTCF_REQUIRE(n >= 0 && (n == 0 || (ops && dense_base && dense_pitch)), "blockmove: bad lists");
+    std::vector<BOp> hops(n);
+    std::vector<Place> hpl(n);
+    std::vector<PackWork> hw;
+    int64_t span = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const dtcf_block_op &o = ops[i];
+        DTCF_REQUIRE(o.rows.nd >= 1 && o.rows.nd <= 3 && o.cols.nd >= 1 && o.cols.nd <= 3,
+                     "blockmove: 1..3 digits per axis");
+        for (int d = 0; d < o.rows.nd; d++) DTCF_REQUIRE(o.rows.ext[d] >= 1, "blockmove: extent < 1");
+        for (int d = 0; d < o.cols.nd; d++) DTCF_REQUIRE(o.cols.ext[d] >= 1, "blockmove: extent < 1");
+        DTCF_REQUIRE(o.base >= 0 && dense_base[i] >= 0, "blockmove: negative base");
+        hops[i].base = o.base;
+        hops[i].r = make_axis(o.rows);
+        hops[i].c = make_axis(o.cols);
+        hops[i].rows = axis_len(o.rows);
+        hops[i].cols = axis_len(o.cols);
+        DTCF_REQUIRE(hops[i].rows < (1ll << 31) && hops[i].cols < (1ll << 31), "blockmove: block too long");
+        DTCF_REQUIRE(dense_pitch[i] >= hops[i].cols, "blockmove: pitch shorter than the block's columns");
+        hpl[i] = Place{dense_base[i], dense_pitch[i]};
+        span = std::max(span, dense_base[i] + (hops[i].rows - 1) * dense_pitch[i] + hops[i].cols);
+        for (int64_t r = 0; r < hops[i].rows; r += 8) hw.push_back(PackWork{(int32_t)i, (int32_t)r});
+    }
+    auto *b = new dtcf_blockmove_s();
+    b->n = n;
+    b->nwork = (int64_t)hw.size();
+    b->dense_span = span;
+    hipError_t e = hipSuccess;
+    if (n) e = hipMalloc(&b->ops, n * sizeof(BOp));
+    if (e == hipSuccess && n) e = hipMalloc(&b->places, n * sizeof(Place));
+    if (e == hipSuccess && b->nwork) e = hipMalloc(&b->work, b->nwork * sizeof(PackWork));
+    if (e == hipSuccess && n) e = hipMemcpy(b->ops, hops.data(), n * sizeof(BOp), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(b->places, hpl.data(), n * sizeof(Place), hipMemcpyHostToDevice);
+    if (e == hipSuccess && b->nwork)
+        e = hipMemcpy(b->work, hw.data(), b->nwork * sizeof(PackWork), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        dtcf_blockmove_destroy(b);
+        return hip_fail(e, "blockmove: plan upload");
+    }
+    *out = b;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_blockmove_info(dtcf_blockmove b, int64_t *dense_span)
+{
+    DTCF_REQUIRE(b, "blockmove: null plan");
+    if (dense_span) *dense_span = b->dense_span;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_blockmove_gather_f64(dtcf_blockmove b, const double *sparse, double *dense,
+                                         void *stream)
+{
+    DTCF_REQUIRE(b, "blockmove: null plan");
+    if (b->nwork == 0) return DTCF_OK;
+    DTCF_REQUIRE(sparse && dense, "blockmove: null buffer");
+    hipLaunchKernelGGL(blockmove_kernel<false>, dim3((unsigned)std::min<int64_t>(b->nwork, 65536)),
+                       dim3(256), 0, as_stream(stream), b->ops, b->places, b->work, b->nwork, 0.0,
+                       sparse, dense);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_blockmove_scatter_f64(dtcf_blockmove b, double alpha, const double *dense,
+                                          double *sparse, void *stream)
+{
+    DTCF_REQUIRE(b, "blockmove: null plan");
+    if (b->nwork == 0) return DTCF_OK;
+    DTCF_REQUIRE(sparse && dense, "blockmove: null buffer");
+    hipLaunchKernelGGL(blockmove_kernel<true>, dim3((unsigned)std::min<int64_t>(b->nwork, 65536)),
+                       dim3(256), 0, as_stream(stream), b->ops, b->places, b->work, b->nwork, alpha,
+                       dense, sparse);
     DTCF_HIP(hipGetLastError());
     return DTCF_OK;
 }

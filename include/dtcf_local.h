@@ -308,6 +308,23 @@ int dtcf_grouped_run(dtcf_grouped g, double alpha, const double *A, const double
                      void *stream);
 int dtcf_grouped_destroy(dtcf_grouped g);
 
+/* Block moves (new; the dense-by-irrep form of a block-sparse term): move i
+ * copies the rows x cols block ops[i] of a block-sparse buffer to a dense
+ * buffer at dense_base[i] + r * dense_pitch[i] + c (gather), or adds
+ * alpha * that region into the block (scatter; blocks must not overlap).
+ * Under an abelian group every allowed block of a term's A (B, C) sits in one
+ * dense matrix per irrep of its row (column) indices, so a term becomes one
+ * dense GEMM per irrep between a gather of A and B and a scatter of C. */
+typedef struct dtcf_blockmove_s *dtcf_blockmove;
+int dtcf_blockmove_plan(const dtcf_block_op *ops, const int64_t *dense_base,
+                        const int64_t *dense_pitch, int64_t n, dtcf_blockmove *out);
+/* 1 + the largest dense offset written or read. */
+int dtcf_blockmove_info(dtcf_blockmove b, int64_t *dense_span);
+int dtcf_blockmove_gather_f64(dtcf_blockmove b, const double *sparse, double *dense, void *stream);
+int dtcf_blockmove_scatter_f64(dtcf_blockmove b, double alpha, const double *dense, double *sparse,
+                               void *stream);
+int dtcf_blockmove_destroy(dtcf_blockmove b);
+
 /* ------------------------------------------------------------------ */
 /* 6. Mixed precision with FP64 recombination (new; the reference is    */
 /*    FP64 only) and device-side synthetic inputs                       */

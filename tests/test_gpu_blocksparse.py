@@ -41,7 +41,7 @@ TERMS = [
 ]
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("term", TERMS)
 def test_grouped_small_vs_oracle(dev, term, cfg):
     ck, ci, ak, ai, bk, bi = term
