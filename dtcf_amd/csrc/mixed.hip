@@ -358,12 +358,12 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
 // P = min(L-1, NS) products, so INT32 stays exact for P * 127^2 * Kc < 2^31;
 // every Kc bytes of K (a multiple of 64) the levels are flushed into C.
 //
-// Workgroup: C tile 64 x 128 (wave layout below).  LDS: two stages of 12 * NS fragments of
+// Workgroup: C tile 128 x 128 (wave layout below).  LDS: two stages of 16 * NS fragments of
 // 1 KiB ([operand digit][16-row block][lane][16 B], the fragment-major order
 // of the per-product kernel: lane = row % 16 + 16 * kq holds bytes
 // 16kq..16kq+15 of the 64-byte step of its row).  Each fragment is one
 // direct global->LDS load of 64 lanes x 16 B, issued a step ahead.
-constexpr int OZ_BM = 64, OZ_BN = 128, OZ_BK = 64;
+constexpr int OZ_BM = 128, OZ_BN = 128, OZ_BK = 64;
 
 struct OzArgs {
     const unsigned char *A, *B;  // digit planes [NS][Mp][Kp], [NS][Np][Kp]
@@ -377,20 +377,23 @@ struct OzArgs {
     uint32_t tiles_m, tiles_n;
 };
 
-// Workgroup of 8 waves (2 per SIMD) as 2 x 4, each owning 32 x 32 of the
-// 64 x 128 tile as 2 x 2 MFMA blocks: under 200 registers per wave, so two
-// waves share a SIMD and hide each other's MFMA and LDS latency.  (Two
-// 64 x 64 workgroups per CU measured slower: more L2 and LDS traffic per
-// product.)
-constexpr int OZ_WAVES = 8, OZ_WM = 32, OZ_WN = 32;
+// Workgroup of 8 waves (2 per SIMD) as 4 x 2, each owning 32 x 64 of the
+// 128 x 128 tile as 2 x 4 MFMA blocks; the two LDS stages take all 160 KiB
+// at NS = 5.  The tile is as large as the registers allow (L - 1 INT32
+// level sums per C element) because the kernel's limit is L2 reuse: every
+// workgroup streams 12 * NS KiB of digit planes per K step, and a 64 x 128
+// tile (1.5x the L2 traffic per product) ran 15-25% slower, two 64 x 64
+// workgroups per CU slower still.
+constexpr int OZ_WAVES = 8, OZ_WM = 32, OZ_WN = 64;
+constexpr int OZ_FM = OZ_WM / 16, OZ_FN = OZ_WN / 16;  // MFMA blocks per wave
 
 template <int NS, int L, bool CVIEW>
-__device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][2][2], int64_t m0,
+__device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][OZ_FM][OZ_FN], int64_t m0,
                                          int64_t n0, int wm, int wn, int lane)
 {
     const int fr = lane & 15, fq = lane >> 4;
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < OZ_FN; j++) {
         const int64_t col = n0 + wn * OZ_WN + j * 16 + fr;
         if (col >= g.N) continue;
         const int ebc = g.eb[col];
@@ -398,7 +401,7 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][2][2]
         if constexpr (CVIEW) cof = g.cv.cofs[col];
         else cof = col;
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int64_t row = m0 + wm * OZ_WM + i * 16 + 4 * fq + r;
@@ -465,48 +468,48 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
         }
     };
 
-    i4 acc[L - 1][2][2];
+    i4 acc[L - 1][OZ_FM][OZ_FN];
 #pragma unroll
     for (int l = 0; l < L - 1; l++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+            for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
 
     auto step = [&](const i4 *st) __attribute__((always_inline)) {
-        i4 af[NS][2];
+        i4 af[NS][OZ_FM];
 #pragma unroll
         for (int d = 0; d < NS; d++)
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < OZ_FM; i++)
                 af[d][i] = st[(d * (OZ_BM / 16) + wm * (OZ_WM / 16) + i) * 64 + lane];
         // B digit by digit (the next one read while this one is used); for
-        // digit dj every product (di, dj), di + dj <= L, over the 4 blocks.
+        // digit dj every product (di, dj), di + dj <= L, over the 8 blocks.
         // Two updates of one accumulator -- (di, dj) and (di - 1, dj + 1) --
-        // are about 4 * (NS - 1) MFMAs apart, and the SIMD's second wave
+        // are about 8 * (NS - 1) MFMAs apart, and the SIMD's second wave
         // interleaves its own chains.
-        i4 bcur[2], bnxt[2];
+        i4 bcur[OZ_FN], bnxt[OZ_FN];
 #pragma unroll
-        for (int j = 0; j < 2; j++) bcur[j] = st[(FA + wn * (OZ_WN / 16) + j) * 64 + lane];
+        for (int j = 0; j < OZ_FN; j++) bcur[j] = st[(FA + wn * (OZ_WN / 16) + j) * 64 + lane];
 #pragma unroll
         for (int dj = 1; dj <= NS; dj++) {
             if (dj < NS) {
 #pragma unroll
-                for (int j = 0; j < 2; j++)
+                for (int j = 0; j < OZ_FN; j++)
                     bnxt[j] = st[(FA + dj * (OZ_BN / 16) + wn * (OZ_WN / 16) + j) * 64 + lane];
             }
 #pragma unroll
             for (int di = 1; di <= NS; di++) {
                 if (di + dj > L) continue;
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++)
+                    for (int j = 0; j < OZ_FN; j++)
                         acc[di + dj - 2][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                             af[di - 1][i], bcur[j], acc[di + dj - 2][i][j], 0, 0, 0);
             }
 #pragma unroll
-            for (int j = 0; j < 2; j++) bcur[j] = bnxt[j];
+            for (int j = 0; j < OZ_FN; j++) bcur[j] = bnxt[j];
         }
     };
     int64_t since = 0;
@@ -518,9 +521,9 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
 #pragma unroll
             for (int l = 0; l < L - 1; l++)
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+                    for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
             since = 0;
         }
     };

@@ -636,6 +636,18 @@ void sched_loop(dtcf_server srv)
         const bool changed = sched_pass(srv);
         bool busy = false;
         for (auto &sl : srv->slots) busy |= sl.r != nullptr;
+        if (srv->stop && srv->active.empty() && !srv->waiting.empty() && !changed) {
+            // closing: nothing runs, and what still waits cannot be admitted
+            for (Req *r : srv->waiting) {
+                r->status = DTCF_ERR_BUDGET;
+                r->err = "server closed before the request could be admitted";
+                r->state = 3;
+                r->t_end = now_us(srv->t0);
+                r->p->refs--;
+            }
+            srv->waiting.clear();
+            srv->done_cv.notify_all();
+        }
         if (srv->stop && !busy && srv->waiting.empty() && srv->active.empty()) break;
         if (changed) continue;
         if (busy) {

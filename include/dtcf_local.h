@@ -397,7 +397,8 @@ typedef struct dtcf_program_s *dtcf_program;
  * `nslots` (1..16) non-blocking streams of default priority.  Each slot
  * also keeps a GEMM tail-split workspace (under 80 MB) outside the arena. */
 int dtcf_server_open(int64_t budget, int nslots, dtcf_server *out);
-/* Waits for every admitted request, then frees everything. */
+/* Waits for every admitted request (queued requests that can no longer be
+ * admitted fail with DTCF_ERR_BUDGET), then frees everything. */
 int dtcf_server_close(dtcf_server s);
 /* Resident tensor storage from the arena (zeroed).  Freed by
  * dtcf_server_free or at close. */
