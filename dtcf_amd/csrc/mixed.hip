@@ -381,7 +381,7 @@ struct OzArgs {
 // 128 x 128 tile as 2 x 4 MFMA blocks; the two LDS stages take all 160 KiB
 // at NS = 5.  The tile is as large as the registers allow (L - 1 INT32
 // level sums per C element) because the kernel's limit is L2 reuse: every
-// workgroup streams 12 * NS KiB of digit planes per K step, and a 64 x 128
+// workgroup streams 16 * NS KiB of digit planes per K step, and a 64 x 128
 // tile (1.5x the L2 traffic per product) ran 15-25% slower, two 64 x 64
 // workgroups per CU slower still.
 constexpr int OZ_WAVES = 8, OZ_WM = 32, OZ_WN = 64;
