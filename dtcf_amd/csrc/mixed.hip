@@ -8,14 +8,17 @@
 //   BF16x3: x = s1 + s2 + s3 + r, s_i = bf16(rest)      (BF16 MFMA, FP32 accum)
 //   F16x3 : the same in FP16 after scaling each row by a power of two so its
 //           largest magnitude is near 2^14 (keeps the slices normal)
-//   I8xS  : each row scaled by 2^-e (e = exponent of its max |x| + 1) and cut
-//           into S signed 7-bit digits, q_i = trunc(r_{i-1} * 2^(7i)),
-//           x = 2^e * sum_i q_i 2^(-7i) + O(2^(e-7S))   (INT8 MFMA, INT32 accum)
+//   I8xS  : each row scaled to integers X = rint(x * 2^(8S - 2 - e)) (e the
+//           exponent above the row's max |x|) and X written in base 256 with
+//           balanced signed digits q_1..q_S in [-128, 127] (q_1 within +-65):
+//           x = 2^(e + 2) * sum_i q_i 2^(-8i) + O(2^(e + 1 - 8S))
+//           (INT8 MFMA, INT32 accum)
 // A product of slices (i, j) is one low-precision GEMM whose accumulators
-// are converted to FP64, scaled (2^(ea_r + eb_c - 7(i+j)) for I8, 2^(ea_r +
-// eb_c) for F16) and added to the FP64 C in the epilogue.  INT8 products are
-// exact in INT32 for K < 2^31 / 127^2 = 133,144, so the only I8 errors are
-// the dropped digits and the FP64 recombination (the Ozaki scheme of Ozaki,
+// are converted to FP64, scaled (2^(ea_r + eb_c + 4 - 8(i+j)) for I8,
+// 2^(ea_r + eb_c) for F16) and added to the FP64 C in the epilogue.  INT8
+// products are exact in INT32 for K < 2^31 / 128^2 = 131,072, so the only I8
+// errors are the rounding to 8S - 2 bits, dropped levels and the FP64
+// recombination (the Ozaki scheme of Ozaki,
 // Ogita, Oishi and Rump, "Error-free transformations of matrix
 // multiplication by using fast routines of matrix multiplication and its
 // applications", Numer. Algorithms 59 (2012), with integer slices).
@@ -152,12 +155,16 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
                 r -= (double)(float)q;
             }
         } else {  // I8
+            // X = rint(x * 2^(8 nslice - 2 - e)), |X| < 2^(8 nslice - 2) (exact in
+            // FP64 for nslice <= 6), written in base 256 with balanced digits
+            // in [-128, 127] from the least significant up; the top digit is
+            // then within +-65.
             int8_t *d = (int8_t *)dst;
-            double r = ldexp(x, -e);  // |r| < 1
-            for (int s = 0; s < nslice; s++) {
-                const double t = trunc(ldexp(r, 7));  // |t| <= 127
-                d[s * plane + o] = (int8_t)t;
-                r = ldexp(r, 7) - t;                  // exact; |r| < 1 again
+            int64_t X = (int64_t)rint(ldexp(x, 8 * nslice - 2 - e));
+            for (int s = nslice - 1; s >= 0; s--) {
+                const int64_t q = ((X + 128) & 255) - 128;
+                d[s * plane + o] = (int8_t)q;
+                X = (X - q) >> 8;  // exact: X - q is a multiple of 256
             }
         }
     }
@@ -352,10 +359,10 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
 // All digit products of one C tile in one kernel.  Per K step of 64 bytes
 // the workgroup stages every digit plane of its A rows and B rows in LDS
 // once and runs every product (i, j), i + j <= L, from registers.  Products
-// of one level l = i + j share the scale 2^(ea + eb - 7l), so they sum
+// of one level l = i + j share the scale 2^(ea + eb + 4 - 8l), so they sum
 // exactly in one INT32 accumulator per level: L-1 accumulator sets instead
 // of one FP64 read-modify-write of C per product.  A level holds at most
-// P = min(L-1, NS) products, so INT32 stays exact for P * 127^2 * Kc < 2^31;
+// P = min(L-1, NS) products, so INT32 stays exact for P * 128^2 * Kc < 2^31;
 // every Kc bytes of K (a multiple of 64) the levels are flushed into C.
 //
 // Workgroup: C tile 128 x 128 (wave layout below).  LDS: two stages of 16 * NS fragments of
@@ -410,7 +417,7 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][OZ_FM
                 double v = 0.0;
 #pragma unroll
                 for (int l = L - 2; l >= 0; l--)  // smallest terms first
-                    v += ldexp((double)acc[l][i][j][r], e - 7 * (l + 2));
+                    v += ldexp((double)acc[l][i][j][r], e + 4 - 8 * (l + 2));
                 int64_t addr;
                 if constexpr (CVIEW) addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + cof;
                 else addr = row * g.ldc + cof;
@@ -569,6 +576,7 @@ int oz_dispatch(int ns, int lev, const OzArgs &g, bool cview, hipStream_t s, boo
     case 3 * 16 + 4: return oz_launch<3, 4>(g, cview, s);
     case 4 * 16 + 4: return oz_launch<4, 4>(g, cview, s);
     case 4 * 16 + 5: return oz_launch<4, 5>(g, cview, s);
+    case 4 * 16 + 6: return oz_launch<4, 6>(g, cview, s);
     case 5 * 16 + 5: return oz_launch<5, 5>(g, cview, s);
     case 5 * 16 + 6: return oz_launch<5, 6>(g, cview, s);
     default: *done = false; return DTCF_OK;
@@ -644,7 +652,7 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
 {
     DTCF_REQUIRE(src && v && dst, "split: null argument");
     DTCF_REQUIRE(prec >= DTCF_PREC_F32 && prec <= DTCF_PREC_I8, "split: unknown precision");
-    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(nslice >= 1 && nslice <= 7, "split: 1..7 INT8 slices");
+    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(nslice >= 1 && nslice <= 6, "split: 1..6 INT8 slices");
     if (prec == DTCF_PREC_F16X3 || prec == DTCF_PREC_I8) DTCF_REQUIRE(rexp, "split: needs rexp");
     ViewArgs a{};
     int64_t rows, K;
@@ -700,7 +708,7 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
     const bool scaled = prec == DTCF_PREC_F16X3 || prec == DTCF_PREC_I8;
     if (scaled) DTCF_REQUIRE(ea && eb, "gemm_split: scaled slices need row exponents");
     const int ns = prec == DTCF_PREC_F32 ? 1 : prec == DTCF_PREC_I8 ? nslice : 3;
-    DTCF_REQUIRE(ns >= 1 && ns <= 7, "gemm_split: bad slice count");
+    DTCF_REQUIRE(ns >= 1 && ns <= 6, "gemm_split: bad slice count");
 
     LPArgs g{};
     g.M = m;
@@ -742,23 +750,23 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
         o.ldc = ldc;
         o.tiles_m = (uint32_t)(g.Mp / OZ_BM);
         o.tiles_n = (uint32_t)(g.Np / OZ_BN);
-        // INT32 exact while (products per level) * 127^2 * K < 2^31
+        // INT32 exact while (products per level) * 128^2 * K < 2^31
         const int per_level = std::min(lev - 1, ns);
-        const int64_t kmax = ((int64_t(1) << 31) - 1) / ((int64_t)per_level * 127 * 127);
+        const int64_t kmax = ((int64_t(1) << 31) - 1) / ((int64_t)per_level * 128 * 128);
         o.kchunk_steps = std::max<int64_t>(1, kmax / OZ_BK);
         bool done = false;
         int st = oz_dispatch(ns, lev, o, cv != nullptr, s, &done);
         if (st != DTCF_OK || done) return st;
     }
     if (prec == DTCF_PREC_I8)
-        DTCF_REQUIRE(k < 133144, "gemm_split: INT8 products exact only for K < 133144");
+        DTCF_REQUIRE(k < 131072, "gemm_split: INT8 products exact only for K < 131072");
     for (int i = 1; i <= ns; i++)
         for (int j = 1; j <= ns && i + j <= lev; j++) {
             g.A = (const unsigned char *)A_slices + (i - 1) * pa;
             g.B = (const unsigned char *)B_slices + (j - 1) * pb;
             g.ea = scaled ? ea : nullptr;
             g.eb = scaled ? eb : nullptr;
-            g.shift = prec == DTCF_PREC_I8 ? 7 * (i + j) : 0;
+            g.shift = prec == DTCF_PREC_I8 ? 8 * (i + j) - 4 : 0;
             switch (prec) {
             case DTCF_PREC_F32:
                 cv ? lp_launch<DTCF_PREC_F32, true>(g, s) : lp_launch<DTCF_PREC_F32, false>(g, s);

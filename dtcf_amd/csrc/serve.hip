@@ -773,7 +773,7 @@ extern "C" int dtcf_program_create(dtcf_server s, int prec, int nslice, int max_
 {
     DTCF_REQUIRE(s && out, "program: bad argument");
     DTCF_REQUIRE(prec == 0 || (prec >= DTCF_PREC_F32 && prec <= DTCF_PREC_I8), "program: unknown precision");
-    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(nslice >= 1 && nslice <= 7, "program: 1..7 INT8 slices");
+    if (prec == DTCF_PREC_I8) DTCF_REQUIRE(nslice >= 1 && nslice <= 6, "program: 1..6 INT8 slices");
     auto *p = new dtcf_program_s();
     p->srv = s;
     p->prec = prec;

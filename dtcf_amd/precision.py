@@ -33,6 +33,7 @@ VARIANTS = [
     ("bf16x3", mixed.BF16X3, 3, 0),
     ("fp16x3", mixed.F16X3, 3, 0),
     ("int8x4", mixed.I8, 4, 0),      # 10 products, fused
+    ("int8x4/L6", mixed.I8, 4, 6),   # 13 products, fused
     ("int8x5/L5", mixed.I8, 5, 5),   # 10 products, fused
     ("int8x5", mixed.I8, 5, 0),      # 15 products, fused
     ("int8x6", mixed.I8, 6, 0),      # 21 products, one GEMM each

@@ -333,8 +333,9 @@ enum {
     DTCF_PREC_F32 = 1,    /* one FP32 slice, FP32 MFMA, FP32 accumulation */
     DTCF_PREC_BF16X3 = 2, /* three BF16 slices, BF16 MFMA, FP32 accumulation */
     DTCF_PREC_F16X3 = 3,  /* three FP16 slices of row-scaled values, FP32 accumulation */
-    DTCF_PREC_I8 = 4      /* nslice signed 7-bit INT8 digits of row-scaled values,
-                             INT8 MFMA, exact INT32 accumulation (Ozaki scheme) */
+    DTCF_PREC_I8 = 4      /* nslice (1..6) balanced base-256 INT8 digits of each row
+                             rounded to 8*nslice-2 bits, INT8 MFMA, exact INT32
+                             accumulation (Ozaki scheme) */
 };
 /* Device bytes needed for the slices of a rows x K operand. */
 int64_t dtcf_split_bytes(int prec, int nslice, int64_t rows, int64_t K);
@@ -350,9 +351,10 @@ int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view v, void *d
  * from dtcf_split_f64 over an m x k view, B_slices over an n x k view.  C is
  * a view (cv) or dense row-major with leading dimension ldc. */
 /* INT8 digit products: 1 (default) runs every product of a call in one
- * fused kernel (level sums in INT32, flushed to C every <= 26,624 K) where
+ * fused kernel (level sums in INT32, flushed to C every <= 26,176 K) where
  * a specialisation exists -- (nslice, max_level) in {(3,4), (4,4), (4,5),
- * (5,5), (5,6)}; otherwise, or with 0, one GEMM per product (K < 133,144). */
+ * (4,6), (5,5), (5,6)}; otherwise, or with 0, one GEMM per product
+ * (K < 131,072). */
 int dtcf_gemm_split_set_fused(int on);
 int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k, double alpha,
                         const void *A_slices, const int32_t *ea, const void *B_slices,

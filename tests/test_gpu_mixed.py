@@ -56,7 +56,7 @@ def test_int8_slices_error_decreases(dev):
         mixed.gemm_split(sa, sb, Cd, ldc=n)
         errs.append(relerr(Cd.cpu().numpy(), ref))
     print("int8 slices 2..6:", errs, flush=True)
-    # each extra digit gains a factor close to 2^7
+    # each extra digit gains a factor close to 2^8
     assert all(errs[i + 1] < errs[i] / 50 for i in range(len(errs) - 1))
     assert errs[-1] < 5e-11
 
@@ -100,7 +100,7 @@ def test_device_generators_match_host(dev):
                        rtol=1e-15)
 
 
-@pytest.mark.parametrize("ns,lev", [(5, 6), (4, 5), (5, 5), (3, 4), (4, 4)])
+@pytest.mark.parametrize("ns,lev", [(5, 6), (4, 5), (4, 6), (5, 5), (3, 4), (4, 4)])
 @pytest.mark.parametrize("m,n,k,cview", [(200, 136, 520, False), (64, 300, 96, True),
                                          (130, 260, 27000, False), (70, 130, 27000, True)])
 def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
@@ -133,4 +133,4 @@ def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
     assert d <= 1e-13
     # normwise error vs FP64, dominated by the dropped digits (ns) or, for
     # L = ns, by the dropped level-(ns+1) products
-    assert e <= {(5, 6): 5e-9, (4, 5): 5e-7, (5, 5): 1e-6, (3, 4): 3e-5, (4, 4): 3e-5}[(ns, lev)]
+    assert e <= {(5, 6): 5e-9, (4, 5): 5e-7, (4, 6): 5e-8, (5, 5): 1e-6, (3, 4): 3e-5, (4, 4): 3e-5}[(ns, lev)]
