@@ -94,6 +94,48 @@ int dtcf_gemm_f64(int transa, int transb, int64_t m, int64_t n, int64_t k, doubl
     return gemm_f64_launch(g, pick_config(m, n, k), as_stream(stream));
 }
 
+int dtcf_gemm_group_f64(const dtcf_gemm_problem *probs, int nprob, double alpha, int overwrite,
+                        int cfg, void *stream)
+{
+    DTCF_REQUIRE(nprob >= 0 && (probs || nprob == 0), "gemm group: null problem list");
+    DTCF_REQUIRE(cfg >= 0 && cfg < kCfgCount, "gemm group: unknown tile configuration");
+    std::vector<int> order(nprob);
+    for (int i = 0; i < nprob; i++) {
+        const dtcf_gemm_problem &q = probs[i];
+        DTCF_REQUIRE(q.m > 0 && q.n > 0 && q.k > 0, "gemm group: empty problem");
+        DTCF_REQUIRE(q.A && q.B && q.C, "gemm group: null operand");
+        DTCF_REQUIRE(q.lda >= q.k && q.ldb >= q.k && q.ldc >= q.n, "gemm group: pitch too small");
+        order[i] = i;
+    }
+    // Longest tiles first: the hardware dispatches workgroups in id order.
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return probs[x].k > probs[y].k; });
+    for (int b = 0; b < nprob; b += kGroupMax) {
+        GemmGroupArgs ga{};
+        ga.alpha = alpha;
+        ga.overwrite = overwrite ? 1 : 0;
+        ga.np = std::min(kGroupMax, nprob - b);
+        for (int i = 0; i < ga.np; i++) {
+            const dtcf_gemm_problem &q = probs[order[b + i]];
+            GemmGroupProb &P = ga.p[i];
+            P.m = q.m;
+            P.n = q.n;
+            P.k = q.k;
+            P.A = q.A;
+            P.B = q.B;
+            P.C = q.C;
+            P.lda = q.lda;
+            P.ldb = q.ldb;
+            P.ldc = q.ldc;
+            P.vec2a = ((reinterpret_cast<uintptr_t>(q.A) & 15) == 0) && (q.lda % 2 == 0);
+            P.vec2b = ((reinterpret_cast<uintptr_t>(q.B) & 15) == 0) && (q.ldb % 2 == 0);
+        }
+        const int st = gemm_f64_group_launch(ga, cfg, as_stream(stream));
+        if (st != DTCF_OK) return st;
+    }
+    return DTCF_OK;
+}
+
 int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int64_t *cofs,
                      const int64_t *blk, int64_t rpb, int64_t cpb, dtcf_view *out)
 {

@@ -209,8 +209,11 @@ struct Occ {
     static constexpr int v = (BM * BN <= 192 * 96) ? 2 : 1;
 };
 
+// One workgroup's share of a GEMM: logical tile q over K steps [kt0, kt1);
+// a split slice (split) writes raw partial sums to workspace slot sidx.
 template <int BM, int BN, int WM, int WN, int AMODE, int BMODE, bool CVIEW>
-__global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs g)
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, uint32_t q, int64_t kt0, int64_t kt1,
+                                          bool split, uint32_t sidx)
 {
     static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -221,30 +224,9 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave / (BN / WN), wn = wave % (BN / WN);
+    const int64_t K = g.k;
 
     // Logical tile q -> (tm, tn) through a grouped (8 tile rows) swizzle.
-    // Workgroups below t_full own whole tiles; their ids are first remapped
-    // so consecutive tiles land on one XCD (its own L2).  Workgroups above
-    // own one K slice of a remainder tile.
-    const uint32_t bid = blockIdx.x;
-    uint32_t q;
-    int64_t kt0 = 0, kt1 = 0;
-    const int64_t K = g.k;
-    const int64_t nk_all = (K + BK - 1) / BK;
-    bool split = false;
-    uint32_t sidx = 0;
-    if (bid < g.t_full) {
-        q = bid;
-        if ((g.t_full & 7u) == 0) q = (bid & 7u) * (g.t_full >> 3) + (bid >> 3);
-        kt1 = nk_all;
-    } else {
-        split = true;
-        sidx = bid - g.t_full;
-        q = g.t_full + sidx / g.sk;
-        const uint32_t sl = sidx % g.sk;
-        kt0 = (int64_t)sl * g.kchunk;
-        kt1 = min(nk_all, kt0 + g.kchunk);
-    }
     const uint32_t GROUP = 8;
     const uint32_t width = GROUP * g.tiles_n;
     const uint32_t gid = q / width;
@@ -379,6 +361,66 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
                 else g.C[addr] += alpha * acc[i][j][r];
             }
         }
+}
+
+template <int BM, int BN, int WM, int WN, int AMODE, int BMODE, bool CVIEW>
+__global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs g)
+{
+    // Workgroups below t_full own whole tiles; their ids are first remapped
+    // so consecutive tiles land on one XCD (its own L2).  Workgroups above
+    // own one K slice of a remainder tile.
+    const uint32_t bid = blockIdx.x;
+    const int64_t nk_all = (g.k + BK - 1) / BK;
+    uint32_t q, sidx = 0;
+    int64_t kt0 = 0, kt1 = nk_all;
+    bool split = false;
+    if (bid < g.t_full) {
+        q = bid;
+        if ((g.t_full & 7u) == 0) q = (bid & 7u) * (g.t_full >> 3) + (bid >> 3);
+    } else {
+        split = true;
+        sidx = bid - g.t_full;
+        q = g.t_full + sidx / g.sk;
+        kt0 = (int64_t)(sidx % g.sk) * g.kchunk;
+        kt1 = min(nk_all, kt0 + g.kchunk);
+    }
+    gemm_tile<BM, BN, WM, WN, AMODE, BMODE, CVIEW>(g, q, kt0, kt1, split, sidx);
+}
+
+// Several independent dense GEMMs C_p (+)= alpha A_p B_p^T in one launch.
+// Problem p owns workgroups [wg0_p, wg0_p + tiles_p rounded up to 8); wg0_p
+// is a multiple of 8, so a workgroup's XCD is its local id mod 8 and each
+// problem gets the same XCD-contiguous tile order as a lone launch.
+// Workgroups past a problem's last tile exit at once.
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_group_kernel(GemmGroupArgs ga)
+{
+    const uint32_t bid = blockIdx.x;
+    int p = 0;
+    for (int i = 1; i < ga.np; i++)
+        if (bid >= ga.p[i].wg0) p = i;
+    const GemmGroupProb &P = ga.p[p];
+    const uint32_t T = P.tiles_m * P.tiles_n, T8 = (T + 7u) & ~7u;
+    const uint32_t local = bid - P.wg0;
+    const uint32_t q = (local & 7u) * (T8 >> 3) + (local >> 3);
+    if (q >= T) return;  // whole workgroup: no barrier has been reached
+    GemmArgs g{};
+    g.m = P.m;
+    g.n = P.n;
+    g.k = P.k;
+    g.alpha = ga.alpha;
+    g.A = P.A;
+    g.B = P.B;
+    g.C = P.C;
+    g.a.ld = P.lda;
+    g.a.vec2 = P.vec2a;
+    g.b.ld = P.ldb;
+    g.b.vec2 = P.vec2b;
+    g.c.ld = P.ldc;
+    g.tiles_m = P.tiles_m;
+    g.tiles_n = P.tiles_n;
+    g.overwrite = ga.overwrite;
+    gemm_tile<BM, BN, WM, WN, kDK, kDK, false>(g, q, 0, (P.k + BK - 1) / BK, false, 0);
 }
 
 // Sums the K slices of each remainder tile in slice order and adds
@@ -533,7 +575,53 @@ int launch_cfg(GemmArgs g, hipStream_t s)
     return DTCF_ERR_INVALID;
 }
 
+template <int BM, int BN, int WM, int WN>
+int launch_group(GemmGroupArgs ga, hipStream_t s)
+{
+    auto kern = gemm_f64_group_kernel<BM, BN, WM, WN>;
+    constexpr size_t lds = (size_t)2 * (BM + BN) * BK * sizeof(double);
+    if constexpr (lds > 65536) {
+        static std::once_flag once;
+        static hipError_t attr_err = hipSuccess;
+        std::call_once(once, [&] {
+            attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        });
+        DTCF_HIP(attr_err);
+    }
+    uint64_t wg = 0;
+    for (int i = 0; i < ga.np; i++) {
+        GemmGroupProb &P = ga.p[i];
+        P.tiles_m = (uint32_t)((P.m + BM - 1) / BM);
+        P.tiles_n = (uint32_t)((P.n + BN - 1) / BN);
+        P.wg0 = (uint32_t)wg;
+        wg += ((uint64_t)P.tiles_m * P.tiles_n + 7) & ~7ull;
+    }
+    DTCF_REQUIRE(wg < (1ull << 31), "gemm group: too many tiles");
+    if (wg == 0) return DTCF_OK;
+    hipLaunchKernelGGL(kern, dim3((unsigned)wg), dim3(NT), lds, s, ga);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
 }  // namespace
+
+int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, hipStream_t s)
+{
+    DTCF_REQUIRE(ga.np >= 0 && ga.np <= kGroupMax, "gemm group: too many problems");
+    for (int i = 0; i < ga.np; i++)
+        DTCF_REQUIRE(ga.p[i].m > 0 && ga.p[i].n > 0 && ga.p[i].k > 0, "gemm group: empty problem");
+    if (ga.np == 0) return DTCF_OK;
+    switch (cfg) {
+    case kCfg128x128: return launch_group<128, 128, 64, 64>(ga, s);
+    case kCfg192x96: return launch_group<192, 96, 96, 48>(ga, s);
+    case kCfg96x192: return launch_group<96, 192, 48, 96>(ga, s);
+    case kCfg64x64: return launch_group<64, 64, 32, 32>(ga, s);
+    default: break;
+    }
+    set_error("gemm group: unknown tile configuration");
+    return DTCF_ERR_INVALID;
+}
 
 int gemm_reserve_workspace(hipStream_t s)
 {

@@ -50,6 +50,28 @@ struct GemmArgs {
 
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
 
+// Up to kGroupMax independent dense GEMMs C_p (+)= alpha * A_p * B_p^T in
+// one launch (A_p m x k, B_p n x k, both k-contiguous; C_p row-major).
+// tiles_m, tiles_n and wg0 are filled in by the launch.
+constexpr int kGroupMax = 16;
+struct GemmGroupProb {
+    int64_t m, n, k;
+    const double *A;
+    const double *B;
+    double *C;
+    int64_t lda, ldb, ldc;
+    int32_t vec2a, vec2b;
+    uint32_t tiles_m, tiles_n, wg0;
+};
+struct GemmGroupArgs {
+    double alpha;
+    int32_t overwrite;
+    int32_t np;
+    GemmGroupProb p[kGroupMax];
+};
+
+int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, hipStream_t s);
+
 // Allocates the tail-split workspace of stream `s` at the largest size any
 // GEMM on it can need, so later launches on `s` never allocate.
 int gemm_reserve_workspace(hipStream_t s);

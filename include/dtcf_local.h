@@ -325,6 +325,27 @@ int dtcf_blockmove_scatter_f64(dtcf_blockmove b, double alpha, const double *den
                                void *stream);
 int dtcf_blockmove_destroy(dtcf_blockmove b);
 
+/* Grouped dense GEMMs (new; the per-irrep GEMMs of a dense-by-irrep term in
+ * one launch): for every problem p,
+ *   C_p = alpha * A_p * B_p^T       (overwrite != 0: C_p is not read)
+ *   C_p += alpha * A_p * B_p^T      (overwrite == 0)
+ * with A_p m x k (row pitch lda), B_p n x k (ldb), C_p m x n (ldc), all
+ * row-major device buffers.  Problems must be non-empty and their C regions
+ * disjoint.  cfg selects the C tile (0 = 128x128, 1 = 192x96, 2 = 96x192,
+ * 3 = 64x64); problems are launched in order of decreasing k, 16 at most
+ * per launch. */
+typedef struct {
+    int64_t m, n, k;
+    const double *A;
+    int64_t lda;
+    const double *B;
+    int64_t ldb;
+    double *C;
+    int64_t ldc;
+} dtcf_gemm_problem;
+int dtcf_gemm_group_f64(const dtcf_gemm_problem *probs, int nprob, double alpha, int overwrite,
+                        int cfg, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* 6. Mixed precision with FP64 recombination (new; the reference is    */
 /*    FP64 only) and device-side synthetic inputs                       */

@@ -81,3 +81,24 @@ def test_error_path_without_gpu_calls():
     s = _lib.i32_array([0, 0])
     st = L.dtcf_permute_tiles_f64(None, None, 2, w, s, 1, None)
     assert st == -1 and b"permutation" in L.dtcf_last_error()
+
+
+def test_gemm_group_rejects_bad_problems_without_gpu_calls():
+    """dtcf_gemm_group_f64 validates every problem before launching any."""
+    import ctypes as C
+
+    from dtcf_amd import _lib, blocksparse as bsp
+
+    L = _lib.lib()
+    P = bsp.GemmProblem
+    ok = P(4, 4, 4, 16, 4, 16, 4, 16, 4)      # fake, aligned, never dereferenced
+    empty = P(4, 0, 4, 16, 4, 16, 4, 16, 4)
+    short_pitch = P(4, 4, 8, 16, 4, 16, 8, 16, 4)
+    null_b = P(4, 4, 4, 16, 4, None, 4, 16, 4)
+    for bad, msg in ((empty, b"empty"), (short_pitch, b"pitch"), (null_b, b"null")):
+        arr = (P * 2)(ok, bad)
+        st = L.dtcf_gemm_group_f64(C.cast(arr, C.c_void_p), 2, 1.0, 1, 0, None)
+        assert st == -1 and msg in L.dtcf_last_error(), msg
+    arr = (P * 1)(ok)
+    assert L.dtcf_gemm_group_f64(C.cast(arr, C.c_void_p), 1, 1.0, 1, 9, None) == -1
+    assert L.dtcf_gemm_group_f64(None, 0, 1.0, 1, 0, None) == 0

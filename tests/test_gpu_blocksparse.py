@@ -73,13 +73,14 @@ def test_grouped_deterministic_and_alone_vs_mixed(dev):
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
 
 
-def test_grouped_config3_full_size(dev):
+@pytest.mark.parametrize("cfg", [None, 4])
+def test_grouped_config3_full_size(dev, cfg):
     # config 3: o=64, v=256 over 8 irreps, ~4000 uneven block pairs per term
     sp = bsp.Space.seeded(o=64, v=256, seed=5)
     A = bsp.BSTensor(sp, "oovv", dev).fill_normal(1)
     B = bsp.BSTensor(sp, "oovv", dev).fill_normal(2)
     Ct = bsp.BSTensor(sp, "oovv", dev)
-    t = bsp.BlockTerm(Ct, "ijab", A, "ikac", B, "kjcb")
+    t = bsp.BlockTerm(Ct, "ijab", A, "ikac", B, "kjcb", cfg=cfg)
     print("pairs", t.npairs, "work", t.nwork, "flops", t.flops, flush=True)
     assert 3000 <= t.npairs <= 5000
     t.run(1.0)
@@ -87,3 +88,25 @@ def test_grouped_config3_full_size(dev):
     ref = oracle.bs_contract(meta(Ct), np.zeros(Ct.size), "ijab", meta(A), A.data.cpu().numpy(),
                              "ikac", meta(B), B.data.cpu().numpy(), "kjcb", 1.0)
     assert relerr(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("gcfg", [0, 3])
+@pytest.mark.parametrize("term", TERMS[:2])
+def test_irrep_dense_tiles_and_resident_a(dev, term, gcfg):
+    """Dense-by-irrep path with each grouped-GEMM tile forced, then a second
+    run reusing the gathered A: its increment equals the first bitwise."""
+    ck, ci, ak, ai, bk, bi = term
+    sp = bsp.Space.seeded(o=16, v=40, seed=6)
+    A = bsp.BSTensor(sp, ak, dev).fill_normal(1)
+    B = bsp.BSTensor(sp, bk, dev).fill_normal(2)
+    Ct = bsp.BSTensor(sp, ck, dev).fill_normal(3)
+    Ah, Bh, C0 = A.data.cpu().numpy(), B.data.cpu().numpy(), Ct.data.cpu().numpy()
+    t = bsp.BlockTerm(Ct, ci, A, ai, B, bi, cfg=4)
+    t._dense.gemm_cfg = gcfg
+    t.run(0.5)
+    C1 = Ct.data.cpu().numpy()
+    ref = oracle.bs_contract(meta(Ct), C0.copy(), ci, meta(A), Ah, ai, meta(B), Bh, bi, 0.5)
+    assert relerr(C1 - C0, ref - C0) <= TOL
+    Ct.data.copy_(torch.from_numpy(C0))
+    t.run(0.5, a_resident=True)
+    assert np.array_equal(Ct.data.cpu().numpy(), C1)

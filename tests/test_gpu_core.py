@@ -132,6 +132,49 @@ def test_gemm_leading_dims_and_offsets(dev):
     assert np.array_equal(got[:, n:], C0[:, n:])  # untouched padding
 
 
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+def test_gemm_group_f64_vs_oracle(dev, cfg):
+    """Grouped launch: 19 problems (two launches of at most 16), ragged
+    sizes, odd pitches and 8-byte-offset operands (the scalar load path),
+    overwrite and accumulate; every C region checked, padding untouched."""
+    import ctypes as C
+
+    from dtcf_amd import blocksparse as bsp
+    from dtcf_amd._lib import check, lib
+
+    shapes = [(1, 1, 1), (17, 33, 5), (130, 70, 257), (256, 288, 64), (200, 150, 31),
+              (64, 64, 16), (129, 1, 300), (1, 200, 40), (300, 257, 129), (96, 192, 48)]
+    shapes = (shapes * 2)[:19]
+    rng = np.random.default_rng(11 + cfg)
+    for overwrite in (1, 0):
+        probs, keep, refs = [], [], []
+        for i, (m, n, k) in enumerate(shapes):
+            lda, ldb, ldc = k + (i % 3), k + (i % 2), n + 2 * (i % 2)
+            off = i % 2  # odd element offset: operands not 16-byte aligned
+            A = rng.standard_normal(off + m * lda)
+            B = rng.standard_normal(off + n * ldb)
+            C0 = rng.standard_normal(m * ldc)
+            Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+            Cd = torch.from_numpy(C0.copy()).to(dev)
+            keep += [Ad, Bd, Cd]
+            probs.append(bsp.GemmProblem(m, n, k, Ad.data_ptr() + 8 * off, lda,
+                                         Bd.data_ptr() + 8 * off, ldb, Cd.data_ptr(), ldc))
+            ref = C0.copy() if not overwrite else np.zeros_like(C0)
+            if overwrite:
+                ref.reshape(m, ldc)[:, n:] = C0.reshape(m, ldc)[:, n:]
+            oracle.dgemm(0, 1, m, n, k, -1.25, A[off:], lda, B[off:], ldb, ref, ldc)
+            refs.append((Cd, ref, m, n, ldc))
+        arr = (bsp.GemmProblem * len(probs))(*probs)
+        say("gemm group", cfg, overwrite, len(probs))
+        check(lib().dtcf_gemm_group_f64(C.cast(arr, C.c_void_p), len(probs), -1.25, overwrite, cfg,
+                                        C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        torch.cuda.synchronize()
+        for i, (Cd, ref, m, n, ldc) in enumerate(refs):
+            got = Cd.cpu().numpy()
+            assert relerr(got, ref) <= GEMM_TOL, (cfg, overwrite, i, shapes[i])
+            assert np.array_equal(got.reshape(m, ldc)[:, n:], ref.reshape(m, ldc)[:, n:])
+
+
 def test_gemm_repeat_is_bitwise_deterministic(dev):
     m = n = k = 384
     A = torch.from_numpy(RNG.standard_normal((m, k))).to(dev)
