@@ -426,6 +426,62 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][OZ_FM
     }
 }
 
+// One K step of a workgroup: every product (di, dj), di + dj <= L, of the
+// A fragments in sa ([digit][16-row block][lane]) and the B fragments in sb,
+// added to the level sums.  B is taken digit by digit (the next one read
+// while this one is used); two updates of one accumulator -- (di, dj) and
+// (di - 1, dj + 1) -- are about 8 (NS - 1) MFMAs apart, and the SIMD's
+// second wave interleaves its own chains.
+template <int NS, int L>
+__device__ __forceinline__ void oz_step(i4 (&acc)[L - 1][OZ_FM][OZ_FN], const i4 *sa, const i4 *sb,
+                                        int wm, int wn, int lane)
+{
+    i4 af[NS][OZ_FM];
+#pragma unroll
+    for (int d = 0; d < NS; d++)
+#pragma unroll
+        for (int i = 0; i < OZ_FM; i++)
+            af[d][i] = sa[(d * (OZ_BM / 16) + wm * (OZ_WM / 16) + i) * 64 + lane];
+    i4 bcur[OZ_FN], bnxt[OZ_FN];
+#pragma unroll
+    for (int j = 0; j < OZ_FN; j++) bcur[j] = sb[(wn * (OZ_WN / 16) + j) * 64 + lane];
+#pragma unroll
+    for (int dj = 1; dj <= NS; dj++) {
+        if (dj < NS) {
+#pragma unroll
+            for (int j = 0; j < OZ_FN; j++)
+                bnxt[j] = sb[(dj * (OZ_BN / 16) + wn * (OZ_WN / 16) + j) * 64 + lane];
+        }
+#pragma unroll
+        for (int di = 1; di <= NS; di++) {
+            if (di + dj > L) continue;
+#pragma unroll
+            for (int i = 0; i < OZ_FM; i++)
+#pragma unroll
+                for (int j = 0; j < OZ_FN; j++)
+                    acc[di + dj - 2][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                        af[di - 1][i], bcur[j], acc[di + dj - 2][i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < OZ_FN; j++) bcur[j] = bnxt[j];
+    }
+}
+
+// Tile of workgroup blockIdx.x: XCD-aware remap (consecutive tiles on one
+// XCD and its L2), then groups of 8 tile rows.
+__device__ __forceinline__ void oz_tile(const OzArgs &g, int64_t &m0, int64_t &n0)
+{
+    const uint32_t T = g.tiles_m * g.tiles_n;
+    uint32_t q = blockIdx.x;
+    if ((T & 7u) == 0) q = (q & 7u) * (T >> 3) + (q >> 3);
+    const uint32_t width = 8 * g.tiles_n;
+    const uint32_t first = (q / width) * 8;
+    const uint32_t gsz = min(g.tiles_m - first, 8u);
+    const uint32_t tm = first + (q % width) % gsz, tn = (q % width) / gsz;
+    m0 = (int64_t)tm * OZ_BM;
+    n0 = (int64_t)tn * OZ_BN;
+}
+
 template <int NS, int L, bool CVIEW>
 __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
 {
@@ -441,15 +497,8 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wv / (OZ_BN / OZ_WN), wn = wv % (OZ_BN / OZ_WN);
 
-    // tile: XCD-aware remap, then groups of 8 tile rows
-    const uint32_t T = g.tiles_m * g.tiles_n;
-    uint32_t q = blockIdx.x;
-    if ((T & 7u) == 0) q = (q & 7u) * (T >> 3) + (q >> 3);
-    const uint32_t width = 8 * g.tiles_n;
-    const uint32_t first = (q / width) * 8;
-    const uint32_t gsz = min(g.tiles_m - first, 8u);
-    const uint32_t tm = first + (q % width) % gsz, tn = (q % width) / gsz;
-    const int64_t m0 = (int64_t)tm * OZ_BM, n0 = (int64_t)tn * OZ_BN;
+    int64_t m0, n0;
+    oz_tile(g, m0, n0);
 
     const size_t pa = (size_t)g.Mp * g.Kp, pb = (size_t)g.Np * g.Kp;
     // Fragment f = wave * PER + c comes from a wave-uniform base (scalar
@@ -484,40 +533,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
             for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
 
     auto step = [&](const i4 *st) __attribute__((always_inline)) {
-        i4 af[NS][OZ_FM];
-#pragma unroll
-        for (int d = 0; d < NS; d++)
-#pragma unroll
-            for (int i = 0; i < OZ_FM; i++)
-                af[d][i] = st[(d * (OZ_BM / 16) + wm * (OZ_WM / 16) + i) * 64 + lane];
-        // B digit by digit (the next one read while this one is used); for
-        // digit dj every product (di, dj), di + dj <= L, over the 8 blocks.
-        // Two updates of one accumulator -- (di, dj) and (di - 1, dj + 1) --
-        // are about 8 * (NS - 1) MFMAs apart, and the SIMD's second wave
-        // interleaves its own chains.
-        i4 bcur[OZ_FN], bnxt[OZ_FN];
-#pragma unroll
-        for (int j = 0; j < OZ_FN; j++) bcur[j] = st[(FA + wn * (OZ_WN / 16) + j) * 64 + lane];
-#pragma unroll
-        for (int dj = 1; dj <= NS; dj++) {
-            if (dj < NS) {
-#pragma unroll
-                for (int j = 0; j < OZ_FN; j++)
-                    bnxt[j] = st[(FA + dj * (OZ_BN / 16) + wn * (OZ_WN / 16) + j) * 64 + lane];
-            }
-#pragma unroll
-            for (int di = 1; di <= NS; di++) {
-                if (di + dj > L) continue;
-#pragma unroll
-                for (int i = 0; i < OZ_FM; i++)
-#pragma unroll
-                    for (int j = 0; j < OZ_FN; j++)
-                        acc[di + dj - 2][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                            af[di - 1][i], bcur[j], acc[di + dj - 2][i][j], 0, 0, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < OZ_FN; j++) bcur[j] = bnxt[j];
-        }
+        oz_step<NS, L>(acc, st, st + FA * 64, wm, wn, lane);
     };
     int64_t since = 0;
     auto after = [&](bool more) __attribute__((always_inline)) {
@@ -551,6 +567,113 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
     oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
 }
 
+// The same computation with a deeper load pipeline, for NS <= 4: LDS holds
+// five half stages of NS * 8 fragments (32 KiB at NS = 4), one A or one B
+// tile of a K step each, as five objects h0..h4.  Half n of the stream
+// (A_k = half 2k, B_k = half 2k+1) sits in slot n mod 5; step k computes from
+// slots 2k, 2k+1 and issues B_{k+1} and A_{k+2} into the two slots step k-1
+// read, so A is loaded two steps ahead and B one (the two-stage kernel gives
+// both one).  The end-of-step wait leaves this wave's A_{k+2} loads in
+// flight (vmcnt = NS): direct loads complete in issue order.  The loop is
+// unrolled by five so every slot is a fixed object and the compiler sees
+// that loads into one slot never alias reads of another.
+template <int NS, int L, bool CVIEW>
+__global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
+{
+    static_assert(OZ_BM == OZ_BN, "A and B halves have the same size");
+    constexpr int FH = NS * (OZ_BM / 16);  // fragments per half stage
+    static_assert(FH % OZ_WAVES == 0 && NS <= 4, "five half stages fit LDS for NS <= 4");
+    constexpr int PH = FH / OZ_WAVES;      // direct loads per wave per half (= NS)
+    __shared__ i4 h0[FH * 64];
+    __shared__ i4 h1[FH * 64];
+    __shared__ i4 h2[FH * 64];
+    __shared__ i4 h3[FH * 64];
+    __shared__ i4 h4[FH * 64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wv / (OZ_BN / OZ_WN), wn = wv % (OZ_BN / OZ_WN);
+    int64_t m0, n0;
+    oz_tile(g, m0, n0);
+
+    const size_t pa = (size_t)g.Mp * g.Kp, pb = (size_t)g.Np * g.Kp;
+    const int64_t lane_off = (int64_t)(lane & 15) * g.Kp + 16 * (lane >> 4);
+    // fragment f = wave * PH + c of a half: digit f / 8, 16-row block f % 8
+    auto issue = [&](i4 *slot, const unsigned char *planes, size_t pstride, int64_t row0,
+                     int64_t kt) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < PH; c++) {
+            const int f = wv * PH + c, d = f / (OZ_BM / 16), b = f % (OZ_BM / 16);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(planes + d * pstride + (row0 + 16 * b) * g.Kp + lane_off + kt * OZ_BK),
+                (__attribute__((address_space(3))) void *)(slot + f * 64), 16, 0, 0);
+        }
+    };
+
+    i4 acc[L - 1][OZ_FM][OZ_FN];
+#pragma unroll
+    for (int l = 0; l < L - 1; l++)
+#pragma unroll
+        for (int i = 0; i < OZ_FM; i++)
+#pragma unroll
+            for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+
+    // s_waitcnt with vmcnt = PH and no expcnt / lgkmcnt wait (GFX9 encoding:
+    // vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[15:14])
+    constexpr int WAIT_PH = (PH & 15) | (7 << 4) | (15 << 8) | ((PH >> 4) << 14);
+    const int64_t nk = g.Kp / OZ_BK;
+    int64_t since = 0;
+    // Workgroup barrier without __syncthreads' release fence, which would
+    // wait for every outstanding load (vmcnt 0) and so for A_{k+2} too.  The
+    // waits are explicit instead: before the barrier each wave has its own
+    // loads for the next step landed (s_waitcnt) and has consumed its reads
+    // of this step's slots (the MFMAs used them); the "memory" clobber keeps
+    // the compiler from moving LDS or load instructions across it.
+    auto barrier = []() __attribute__((always_inline)) { asm volatile("s_barrier" ::: "memory"); };
+    auto after = [&](int64_t k) __attribute__((always_inline)) {
+        if (k + 2 < nk) __builtin_amdgcn_s_waitcnt(WAIT_PH);  // A_{k+2} may stay in flight
+        else __builtin_amdgcn_s_waitcnt(0);
+        barrier();
+        if (++since == g.kchunk_steps && k + 1 < nk) {
+            oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+#pragma unroll
+            for (int l = 0; l < L - 1; l++)
+#pragma unroll
+                for (int i = 0; i < OZ_FM; i++)
+#pragma unroll
+                    for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+            since = 0;
+        }
+    };
+
+    issue(h0, g.A, pa, m0, 0);
+    issue(h1, g.B, pb, n0, 0);
+    if (nk > 1) {
+        issue(h2, g.A, pa, m0, 1);
+        __builtin_amdgcn_s_waitcnt(WAIT_PH);
+    } else {
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    barrier();
+#define OZ_RING_STEP(SA, SB, NB, NA, J)                   \
+    {                                                     \
+        const int64_t k = kt + (J);                       \
+        if (k >= nk) break;                               \
+        if (k + 1 < nk) issue(NB, g.B, pb, n0, k + 1);    \
+        if (k + 2 < nk) issue(NA, g.A, pa, m0, k + 2);    \
+        oz_step<NS, L>(acc, SA, SB, wm, wn, lane);        \
+        after(k);                                         \
+    }
+    for (int64_t kt = 0; kt < nk; kt += 5) {
+        OZ_RING_STEP(h0, h1, h3, h4, 0)
+        OZ_RING_STEP(h2, h3, h0, h1, 1)
+        OZ_RING_STEP(h4, h0, h2, h3, 2)
+        OZ_RING_STEP(h1, h2, h4, h0, 3)
+        OZ_RING_STEP(h3, h4, h1, h2, 4)
+    }
+#undef OZ_RING_STEP
+    oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+}
+
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 // 1: digit products of INT8 splits run in the fused kernel where a
@@ -561,8 +684,13 @@ template <int NS, int L>
 int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
 {
     const unsigned grid = g.tiles_m * g.tiles_n;
-    if (cview) hipLaunchKernelGGL((ozaki_kernel<NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
-    else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+    if constexpr (NS <= 4) {
+        if (cview) hipLaunchKernelGGL((ozaki_ring_kernel<NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+        else hipLaunchKernelGGL((ozaki_ring_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+    } else {
+        if (cview) hipLaunchKernelGGL((ozaki_kernel<NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+        else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+    }
     DTCF_HIP(hipGetLastError());
     return DTCF_OK;
 }
