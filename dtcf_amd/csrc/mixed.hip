@@ -109,6 +109,18 @@ __device__ inline int64_t vaddr(const ViewArgs &v, int64_t r, int64_t c)
 }
 
 // One wave per row: max |x| (for the scaled modes), then the slices.
+// INT8 digit planes are stored fragment-major: the 16 rows x 64 bytes of
+// K of row block rb and K step kt form one contiguous KiB at
+// ((rb * (K_p / 64) + kt) * 64 + lane) * 16, lane = row % 16 + 16 * kq
+// holding bytes 16 kq .. 16 kq + 15 of that row's step -- the MFMA
+// operand order, so one direct 16-byte-per-lane load moves whole 128-byte
+// lines straight into an LDS fragment.
+__host__ __device__ inline int64_t i8_frag_offset(int64_t row, int64_t c, int64_t K_p)
+{
+    const int64_t rb = row >> 4, r = row & 15, kt = c >> 6, kc = c & 63;
+    return (((rb * (K_p >> 6) + kt) << 6) + r + 16 * (kc >> 4)) * 16 + (kc & 15);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ src, ViewArgs v,
                                                    int64_t rows, int64_t K, int64_t rows_p,
@@ -160,10 +172,11 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
             // in [-128, 127] from the least significant up; the top digit is
             // then within +-65.
             int8_t *d = (int8_t *)dst;
+            const int64_t of = i8_frag_offset(row, c, K_p);
             int64_t X = (int64_t)rint(ldexp(x, 8 * nslice - 2 - e));
             for (int s = nslice - 1; s >= 0; s--) {
                 const int64_t q = ((X + 128) & 255) - 128;
-                d[s * plane + o] = (int8_t)q;
+                d[s * plane + of] = (int8_t)q;
                 X = (X - q) >> 8;  // exact: X - q is a multiple of 256
             }
         }
@@ -257,15 +270,26 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
     // Global 16-byte chunks: pass i (0..3), wave w, lane l ->
     // row = 16*g + l%16 with g = 2*w + i/2, byte 64*(i%2) + 16*(l/16) of the
     // tile row.  Each chunk holds 16/VB fragments' worth of lanes.
+    // INT8 planes are fragment-major (i8_frag_offset): the same chunk is
+    // lane l's 16 bytes of fragment (row block m0/16 + grp, 64-byte step
+    // 2 kt + i%2).
     uint4 ra[4], rb[4];
     auto gload = [&](int64_t kt) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int grp = 2 * wave + (i >> 1);
-            const int64_t off = (int64_t)(16 * grp + (lane & 15)) * kb + kt * LBYTES +
-                                64 * (i & 1) + 16 * (lane >> 4);
-            ra[i] = *(const uint4 *)(Ab + off);
-            rb[i] = *(const uint4 *)(Bb + off);
+            if constexpr (PREC == DTCF_PREC_I8) {
+                const int64_t nk64 = g.Kp >> 6;
+                const int64_t fa = (((m0 >> 4) + grp) * nk64 + 2 * kt + (i & 1)) * 1024 + 16 * lane;
+                const int64_t fb = (((n0 >> 4) + grp) * nk64 + 2 * kt + (i & 1)) * 1024 + 16 * lane;
+                ra[i] = *(const uint4 *)((const unsigned char *)g.A + fa);
+                rb[i] = *(const uint4 *)((const unsigned char *)g.B + fb);
+            } else {
+                const int64_t off = (int64_t)(16 * grp + (lane & 15)) * kb + kt * LBYTES +
+                                    64 * (i & 1) + 16 * (lane >> 4);
+                ra[i] = *(const uint4 *)(Ab + off);
+                rb[i] = *(const uint4 *)(Bb + off);
+            }
         }
     };
     auto sstore = [&](unsigned char *sa, unsigned char *sb) {
@@ -504,14 +528,15 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
     // Fragment f = wave * PER + c comes from a wave-uniform base (scalar
     // registers) plus one per-lane offset; it is loaded straight into LDS
     // (16 bytes per lane, lane l at byte 16 l of the fragment).
-    const int64_t lane_off = (int64_t)(lane & 15) * g.Kp + 16 * (lane >> 4);
+    // (fragment-major planes: fragment (row block, K step) is one KiB)
+    const int64_t nk64 = g.Kp / OZ_BK, lane_off = 16 * lane;
     auto frag_base = [&](int f) __attribute__((always_inline)) -> const unsigned char * {
         if (f < FA) {
             const int d = f / (OZ_BM / 16), b = f % (OZ_BM / 16);
-            return g.A + d * pa + (m0 + 16 * b) * g.Kp;
+            return g.A + d * pa + ((m0 >> 4) + b) * nk64 * 1024;
         }
         const int h = f - FA, d = h / (OZ_BN / 16), b = h % (OZ_BN / 16);
-        return g.B + d * pb + (n0 + 16 * b) * g.Kp;
+        return g.B + d * pb + ((n0 >> 4) + b) * nk64 * 1024;
     };
     auto issue = [&](i4 *stage, int64_t kt) __attribute__((always_inline)) {
 #pragma unroll
@@ -519,7 +544,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
             const int f = wv * PER + c;
             if (NF % OZ_WAVES != 0 && f >= NF) break;  // wave-uniform
             __builtin_amdgcn_global_load_lds(
-                (const void *)(frag_base(f) + lane_off + kt * OZ_BK),
+                (const void *)(frag_base(f) + lane_off + kt * 1024),
                 (__attribute__((address_space(3))) void *)(stage + f * 64), 16, 0, 0);
         }
     };
@@ -596,15 +621,16 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
     oz_tile(g, m0, n0);
 
     const size_t pa = (size_t)g.Mp * g.Kp, pb = (size_t)g.Np * g.Kp;
-    const int64_t lane_off = (int64_t)(lane & 15) * g.Kp + 16 * (lane >> 4);
+    const int64_t nk64 = g.Kp / OZ_BK, lane_off = 16 * lane;
     // fragment f = wave * PH + c of a half: digit f / 8, 16-row block f % 8
+    // (fragment-major planes: fragment (row block, K step) is one KiB)
     auto issue = [&](i4 *slot, const unsigned char *planes, size_t pstride, int64_t row0,
                      int64_t kt) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < PH; c++) {
             const int f = wv * PH + c, d = f / (OZ_BM / 16), b = f % (OZ_BM / 16);
             __builtin_amdgcn_global_load_lds(
-                (const void *)(planes + d * pstride + (row0 + 16 * b) * g.Kp + lane_off + kt * OZ_BK),
+                (const void *)(planes + d * pstride + (((row0 >> 4) + b) * nk64 + kt) * 1024 + lane_off),
                 (__attribute__((address_space(3))) void *)(slot + f * 64), 16, 0, 0);
         }
     };
