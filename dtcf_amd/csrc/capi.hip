@@ -217,6 +217,8 @@ int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int6
     v->span = span;
     v->colpairs = colpairs;
     v->rowpairs = rowpairs;
+    v->colunit = !blk;
+    for (int64_t c = 1; v->colunit && c < cols; c++) v->colunit = cofs[c] == cofs[0] + c;
     hipError_t e = hipMalloc(&v->rofs, rows * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&v->cofs, cols * sizeof(int64_t));
     if (e == hipSuccess && blk) e = hipMalloc(&v->blk, v->nblk * sizeof(int64_t));
@@ -270,6 +272,7 @@ int dtcf_view_sub(dtcf_view p, int64_t r0, int64_t rows, int64_t c0, int64_t col
     }
     v->colpairs = p->colpairs && c0 % 2 == 0 && cols % 2 == 0;
     v->rowpairs = p->rowpairs && r0 % 2 == 0 && rows % 2 == 0;
+    v->colunit = p->colunit;
     *out = v;
     return DTCF_OK;
 }

@@ -34,6 +34,7 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef int i4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
@@ -121,6 +122,24 @@ __host__ __device__ inline int64_t i8_frag_offset(int64_t row, int64_t c, int64_
     return (((rb * (K_p >> 6) + kt) << 6) + r + 16 * (kc >> 4)) * 16 + (kc & 15);
 }
 
+// Balanced base-256 digits of X = rint(x 2^sh) (|X| < 2^46), least
+// significant first: q = ((X + 128) mod 256) - 128 = X - 256 Y and the
+// next X = Y = floor((X + 128) / 256).  Every step is exact in FP64 (X is
+// an integer below 2^53; 1/256 is a power of two), which keeps the digit
+// loop on full-rate FP64 instructions instead of emulated 64-bit integer
+// ones.  q[s] is digit s, s = 0 the most significant.
+__device__ __forceinline__ void i8_digits(double x, int sh, int nslice, int (&q)[6])
+{
+    double X = rint(ldexp(x, sh));
+#pragma unroll
+    for (int s = 5; s >= 0; s--) {
+        if (s >= nslice) continue;
+        const double Y = floor((X + 128.0) * (1.0 / 256.0));
+        q[s] = (int)(X - 256.0 * Y);
+        X = Y;
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ src, ViewArgs v,
                                                    int64_t rows, int64_t K, int64_t rows_p,
@@ -173,13 +192,103 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
             // then within +-65.
             int8_t *d = (int8_t *)dst;
             const int64_t of = i8_frag_offset(row, c, K_p);
-            int64_t X = (int64_t)rint(ldexp(x, 8 * nslice - 2 - e));
-            for (int s = nslice - 1; s >= 0; s--) {
-                const int64_t q = ((X + 128) & 255) - 128;
-                d[s * plane + of] = (int8_t)q;
-                X = (X - q) >> 8;  // exact: X - q is a multiple of 256
-            }
+            int q[6] = {0, 0, 0, 0, 0, 0};
+            i8_digits(x, 8 * nslice - 2 - e, nslice, q);
+            for (int s = 0; s < nslice; s++) d[s * plane + of] = (int8_t)q[s];
         }
+    }
+}
+
+// INT8 split of a view whose rows are contiguous and 16-byte aligned (no
+// block table, cofs[c] = cofs[0] + c, even offsets).  A workgroup owns one
+// block of 16 rows (the rows of one fragment), wave w rows 4w .. 4w+3.
+// Every load instruction reads 2 KiB of one row (coalesced, one page);
+// per chunk of 256 elements (4 K steps) each wave turns its four rows into
+// digits, four elements per lane, and drops the 4-byte words into an LDS
+// copy of the chunk's fragments; the workgroup then writes each plane's
+// four fragments (4 KiB, contiguous in the fragment-major layout) with
+// 16-byte stores.  Same digits as split_kernel.
+__global__ __launch_bounds__(256) void split_i8_rows_kernel(const double *__restrict__ src, ViewArgs v,
+                                                           int64_t rows, int64_t K, int64_t rows_p,
+                                                           int64_t K_p, int nslice,
+                                                           int8_t *__restrict__ dst,
+                                                           int32_t *__restrict__ rexp)
+{
+    __shared__ uint4 stage[6 * 256];  // [plane][K step of the chunk][fragment lane]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t rb = blockIdx.x;
+    const double *xr[4];
+    bool real[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int64_t row = rb * 16 + 4 * w + i;
+        real[i] = row < rows;
+        xr[i] = real[i] ? src + v.rofs[row] + v.cofs[0] : src;
+    }
+    // row maxima: lane covers elements 4 lane + 256 t .. +3 of each row
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t c = 4 * lane; c < K; c += 256) {  // K is even: c + 1 < K
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (!real[i]) continue;
+            const d2 a = *(const d2 *)(xr[i] + c);
+            double t = fmax(fabs(a.x), fabs(a.y));
+            if (c + 2 < K) {
+                const d2 b = *(const d2 *)(xr[i] + c + 2);
+                t = fmax(t, fmax(fabs(b.x), fabs(b.y)));
+            }
+            m[i] = fmax(m[i], t);
+        }
+    }
+    int sh[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double mm = m[i];
+        for (int off = 32; off >= 1; off >>= 1) mm = fmax(mm, __shfl_xor(mm, off, 64));
+        int e = 0;
+        if (mm > 0.0) frexp(mm, &e);
+        if (lane == 0) rexp[rb * 16 + 4 * w + i] = e;
+        sh[i] = 8 * nslice - 2 - e;
+    }
+    const size_t plane16 = (size_t)rows_p * K_p / 16;  // plane size in uint4
+    const int64_t nk = K_p >> 6;
+    uint32_t *st32 = (uint32_t *)stage;
+    // this lane's four elements in the chunk: K step lane / 16, bytes
+    // 4 (lane % 16) .. +3 of it, i.e. kq = (lane % 16) / 4, byte 4 (lane % 4)
+    const int ks = lane >> 4, kq = (lane & 15) >> 2, byte = 4 * (lane & 3);
+    for (int64_t c0 = 0; c0 < K_p; c0 += 256) {
+        const int64_t c = c0 + 4 * lane;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            d2 a = {0.0, 0.0}, b = {0.0, 0.0};
+            if (real[i] && c < K) {
+                a = *(const d2 *)(xr[i] + c);
+                if (c + 2 < K) b = *(const d2 *)(xr[i] + c + 2);
+            }
+            const double xs[4] = {a.x, a.y, b.x, b.y};
+            uint32_t wd[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int q[6] = {0, 0, 0, 0, 0, 0};
+                i8_digits(xs[j], sh[i], nslice, q);
+#pragma unroll
+                for (int s = 0; s < 6; s++) wd[s] |= (uint32_t)(q[s] & 255) << (8 * j);
+            }
+            const int r = 4 * w + i;
+            const int o = ((ks * 64 + r + 16 * kq) * 16 + byte) >> 2;
+#pragma unroll
+            for (int s = 0; s < 6; s++)
+                if (s < nslice) st32[s * 1024 + o] = wd[s];
+        }
+        __syncthreads();
+        const int64_t kt0 = c0 >> 6;
+        if (kt0 + (tid >> 6) < nk) {  // K_p may end halfway through a chunk
+            uint4 *out = (uint4 *)dst + (rb * nk + kt0) * 64 + tid;
+#pragma unroll
+            for (int s = 0; s < 6; s++)
+                if (s < nslice) out[s * plane16] = stage[s * 256 + tid];
+        }
+        __syncthreads();
     }
 }
 
@@ -829,8 +938,12 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
                            K, rp, kp, 3, dst, rexp);
         break;
     default:
-        hipLaunchKernelGGL(split_kernel<DTCF_PREC_I8>, dim3(grid), dim3(256), 0, s, src, a, rows, K,
-                           rp, kp, nslice, dst, rexp);
+        if (v->colunit && v->colpairs && (reinterpret_cast<uintptr_t>(src) & 15) == 0)
+            hipLaunchKernelGGL(split_i8_rows_kernel, dim3((unsigned)(rp / 16)), dim3(256), 0, s, src, a,
+                               rows, K, rp, kp, nslice, (int8_t *)dst, rexp);
+        else
+            hipLaunchKernelGGL(split_kernel<DTCF_PREC_I8>, dim3(grid), dim3(256), 0, s, src, a, rows,
+                               K, rp, kp, nslice, dst, rexp);
         break;
     }
     DTCF_HIP(hipGetLastError());

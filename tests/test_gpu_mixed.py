@@ -134,3 +134,37 @@ def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
     # normwise error vs FP64, dominated by the dropped digits (ns) or, for
     # L = ns, by the dropped level-(ns+1) products
     assert e <= {(5, 6): 5e-9, (4, 5): 5e-7, (4, 6): 5e-8, (5, 5): 1e-6, (3, 4): 3e-5, (4, 4): 3e-5}[(ns, lev)]
+
+
+@pytest.mark.parametrize("ns", [4, 5, 6])
+@pytest.mark.parametrize("rows,K", [(200, 522), (128, 256), (17, 66)])
+def test_int8_split_layout_and_digits(dev, ns, rows, K):
+    """The contiguous-row split (16-byte loads) and the generic one (same
+    data behind a one-block table) write identical planes; decoded from the
+    fragment-major layout, the digits lie in [-128, 127] and reconstruct
+    X = rint(x 2^(8ns-2-e)) with e the exponent of the row max (frexp)."""
+    rng = np.random.default_rng(rows * 7 + K + ns)
+    A = rng.standard_normal((rows, K)) * np.exp2(rng.integers(-20, 20, size=(rows, 1)))
+    A[3 % rows] = 0.0  # an all-zero row: e = 0, all digits 0
+    Ad = torch.from_numpy(A).to(dev)
+    fast = mixed.Split(mixed.I8, Ad, mixed.dense_view(rows, K), ns)
+    gen_view = ops.View(rows, K, np.arange(rows) * K, np.arange(K), blk=np.zeros(1, np.int64),
+                        rpb=rows, cpb=K)
+    gen = mixed.Split(mixed.I8, Ad, gen_view, ns)
+    torch.cuda.synchronize()
+    assert torch.equal(fast.buf, gen.buf) and torch.equal(fast.rexp, gen.rexp)
+    rp, kp = (rows + 127) // 128 * 128, (K + 127) // 128 * 128
+    planes = fast.buf.cpu().numpy().view(np.int8).reshape(ns, rp // 16, kp // 64, 4, 16, 16)
+    # [digit][row block][K step][kq][row % 16][byte] -> [digit][row][k]
+    q = planes.transpose(0, 1, 4, 2, 3, 5).reshape(ns, rp, kp).astype(np.int64)
+    assert q.min() >= -128 and q.max() <= 127
+    X = np.zeros((rp, kp), dtype=np.int64)
+    for s in range(ns):
+        X = X * 256 + q[s]
+    e = fast.rexp.cpu().numpy()[:rows]
+    for r in range(rows):
+        m = np.max(np.abs(A[r]))
+        assert e[r] == (np.frexp(m)[1] if m > 0 else 0)
+    target = np.ldexp(A, (8 * ns - 2 - e)[:, None])
+    assert np.max(np.abs(X[:rows, :K] - target)) <= 0.5
+    assert not X[rows:].any() and not X[:, K:].any()  # zero padding
