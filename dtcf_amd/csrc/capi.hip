@@ -217,8 +217,8 @@ int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int6
     v->span = span;
     v->colpairs = colpairs;
     v->rowpairs = rowpairs;
-    v->colunit = !blk;
-    for (int64_t c = 1; v->colunit && c < cols; c++) v->colunit = cofs[c] == cofs[0] + c;
+    v->rowrun16 = !blk;
+    for (int64_t r = 0; v->rowrun16 && r < rows; r++) v->rowrun16 = rofs[r] == rofs[r & ~15ll] + (r & 15);
     hipError_t e = hipMalloc(&v->rofs, rows * sizeof(int64_t));
     if (e == hipSuccess) e = hipMalloc(&v->cofs, cols * sizeof(int64_t));
     if (e == hipSuccess && blk) e = hipMalloc(&v->blk, v->nblk * sizeof(int64_t));
@@ -272,7 +272,7 @@ int dtcf_view_sub(dtcf_view p, int64_t r0, int64_t rows, int64_t c0, int64_t col
     }
     v->colpairs = p->colpairs && c0 % 2 == 0 && cols % 2 == 0;
     v->rowpairs = p->rowpairs && r0 % 2 == 0 && rows % 2 == 0;
-    v->colunit = p->colunit;
+    v->rowrun16 = p->rowrun16 && r0 % 16 == 0;
     *out = v;
     return DTCF_OK;
 }

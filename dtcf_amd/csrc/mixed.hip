@@ -199,10 +199,11 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
     }
 }
 
-// INT8 split of a view whose rows are contiguous and 16-byte aligned (no
-// block table, cofs[c] = cofs[0] + c, even offsets).  A workgroup owns one
+// INT8 split of a view without a block table whose column pairs are
+// contiguous and 16-byte aligned (colpairs).  A workgroup owns one
 // block of 16 rows (the rows of one fragment), wave w rows 4w .. 4w+3.
-// Every load instruction reads 2 KiB of one row (coalesced, one page);
+// Every load instruction reads 256 elements of one row (2 KiB when the
+// row is contiguous);
 // per chunk of 256 elements (4 K steps) each wave turns its four rows into
 // digits, four elements per lane, and drops the 4-byte words into an LDS
 // copy of the chunk's fragments; the workgroup then writes each plane's
@@ -223,18 +224,20 @@ __global__ __launch_bounds__(256) void split_i8_rows_kernel(const double *__rest
     for (int i = 0; i < 4; i++) {
         const int64_t row = rb * 16 + 4 * w + i;
         real[i] = row < rows;
-        xr[i] = real[i] ? src + v.rofs[row] + v.cofs[0] : src;
+        xr[i] = real[i] ? src + v.rofs[row] : src;
     }
     // row maxima: lane covers elements 4 lane + 256 t .. +3 of each row
+    // (the pairs at cofs[c] and cofs[c + 2])
     double m[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t c = 4 * lane; c < K; c += 256) {  // K is even: c + 1 < K
+        const int64_t o0 = v.cofs[c], o2 = c + 2 < K ? v.cofs[c + 2] : 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             if (!real[i]) continue;
-            const d2 a = *(const d2 *)(xr[i] + c);
+            const d2 a = *(const d2 *)(xr[i] + o0);
             double t = fmax(fabs(a.x), fabs(a.y));
             if (c + 2 < K) {
-                const d2 b = *(const d2 *)(xr[i] + c + 2);
+                const d2 b = *(const d2 *)(xr[i] + o2);
                 t = fmax(t, fmax(fabs(b.x), fabs(b.y)));
             }
             m[i] = fmax(m[i], t);
@@ -258,12 +261,13 @@ __global__ __launch_bounds__(256) void split_i8_rows_kernel(const double *__rest
     const int ks = lane >> 4, kq = (lane & 15) >> 2, byte = 4 * (lane & 3);
     for (int64_t c0 = 0; c0 < K_p; c0 += 256) {
         const int64_t c = c0 + 4 * lane;
+        const int64_t o0 = c < K ? v.cofs[c] : 0, o2 = c + 2 < K ? v.cofs[c + 2] : 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             d2 a = {0.0, 0.0}, b = {0.0, 0.0};
             if (real[i] && c < K) {
-                a = *(const d2 *)(xr[i] + c);
-                if (c + 2 < K) b = *(const d2 *)(xr[i] + c + 2);
+                a = *(const d2 *)(xr[i] + o0);
+                if (c + 2 < K) b = *(const d2 *)(xr[i] + o2);
             }
             const double xs[4] = {a.x, a.y, b.x, b.y};
             uint32_t wd[6] = {0, 0, 0, 0, 0, 0};
@@ -289,6 +293,69 @@ __global__ __launch_bounds__(256) void split_i8_rows_kernel(const double *__rest
                 if (s < nslice) out[s * plane16] = stage[s * 256 + tid];
         }
         __syncthreads();
+    }
+}
+
+// INT8 split of a view without a block table whose rows are contiguous in
+// runs of 16 (rofs[r] = rofs[r & ~15] + r % 16: a row-major operand read
+// transposed, as the ring's integrals and the ladder's amplitudes are).
+// One workgroup of 16 waves per block of 16 rows, wave w on the w-th
+// sixteenth of the K steps; lane l = r + 16 kq on row r.  Each load
+// instruction reads 16 consecutive rows of 4 columns (whole cache lines);
+// the row maxima of the 16 waves meet in LDS; then lane l converts row r's
+// elements 16 kq .. 16 kq + 15 of each of its K steps -- its own lane of
+// the fragment -- so each plane gets one 16-byte store per lane and whole
+// 1 KiB fragments.  Same digits as split_kernel.
+__global__ __launch_bounds__(1024) void split_i8_cols_kernel(const double *__restrict__ src, ViewArgs v,
+                                                            int64_t rows, int64_t K, int64_t rows_p,
+                                                            int64_t K_p, int nslice,
+                                                            int8_t *__restrict__ dst,
+                                                            int32_t *__restrict__ rexp)
+{
+    __shared__ double smax[16][16];  // [wave][row of the block]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
+    const int64_t rb = blockIdx.x;
+    const int64_t row = rb * 16 + r;
+    const bool real = row < rows;
+    const double *x = real ? src + v.rofs[rb * 16] + r : src;  // rofs[row] = rofs[rb*16] + r
+    const int64_t nk = K_p >> 6, per = (nk + 15) / 16;
+    const int64_t kt0 = min(nk, w * per), kt1 = min(nk, kt0 + per);
+    const int64_t cend = min(K, kt1 * 64);
+    double m0 = 0.0, m1 = 0.0;
+    for (int64_t c = kt0 * 64 + kq; real && c < cend; c += 8) {
+        m0 = fmax(m0, fabs(x[v.cofs[c]]));
+        if (c + 4 < cend) m1 = fmax(m1, fabs(x[v.cofs[c + 4]]));
+    }
+    double m = fmax(m0, m1);
+    m = fmax(m, __shfl_xor(m, 16, 64));
+    m = fmax(m, __shfl_xor(m, 32, 64));
+    if (kq == 0) smax[w][r] = m;
+    __syncthreads();
+    m = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; u++) m = fmax(m, smax[u][r]);
+    int e = 0;
+    if (m > 0.0) frexp(m, &e);
+    if (w == 0 && kq == 0) rexp[row] = e;
+    const size_t plane16 = (size_t)rows_p * K_p / 16;
+    const int sh = 8 * nslice - 2 - e;
+    uint4 *out = (uint4 *)dst + rb * nk * 64 + lane;
+    for (int64_t kt = kt0; kt < kt1; kt++) {
+        const int64_t c = kt * 64 + 16 * kq;
+        double xs[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) xs[j] = (real && c + j < K) ? x[v.cofs[c + j]] : 0.0;
+        uint32_t wd[6][4] = {};
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            int q[6] = {0, 0, 0, 0, 0, 0};
+            i8_digits(xs[j], sh, nslice, q);
+#pragma unroll
+            for (int s = 0; s < 6; s++) wd[s][j >> 2] |= (uint32_t)(q[s] & 255) << (8 * (j & 3));
+        }
+#pragma unroll
+        for (int s = 0; s < 6; s++)
+            if (s < nslice) out[s * plane16 + kt * 64] = make_uint4(wd[s][0], wd[s][1], wd[s][2], wd[s][3]);
     }
 }
 
@@ -938,8 +1005,11 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
                            K, rp, kp, 3, dst, rexp);
         break;
     default:
-        if (v->colunit && v->colpairs && (reinterpret_cast<uintptr_t>(src) & 15) == 0)
+        if (!v->blk && v->colpairs && (reinterpret_cast<uintptr_t>(src) & 15) == 0)
             hipLaunchKernelGGL(split_i8_rows_kernel, dim3((unsigned)(rp / 16)), dim3(256), 0, s, src, a,
+                               rows, K, rp, kp, nslice, (int8_t *)dst, rexp);
+        else if (v->rowrun16)
+            hipLaunchKernelGGL(split_i8_cols_kernel, dim3((unsigned)(rp / 16)), dim3(1024), 0, s, src, a,
                                rows, K, rp, kp, nslice, (int8_t *)dst, rexp);
         else
             hipLaunchKernelGGL(split_kernel<DTCF_PREC_I8>, dim3(grid), dim3(256), 0, s, src, a, rows,

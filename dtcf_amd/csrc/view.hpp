@@ -7,6 +7,6 @@ struct dtcf_view_s {
     int64_t *rofs = nullptr, *cofs = nullptr, *blk = nullptr;  // device
     int64_t span = 0;
     bool colpairs = false, rowpairs = false;
-    bool colunit = false;  // no block table and cofs[c] = cofs[0] + c: rows are contiguous
+    bool rowrun16 = false;  // no block table and rofs[r] = rofs[r & ~15] + (r & 15)
     bool owner = true;  // sub-views share their parent's tables
 };

@@ -139,8 +139,10 @@ def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
 @pytest.mark.parametrize("ns", [4, 5, 6])
 @pytest.mark.parametrize("rows,K", [(200, 522), (128, 256), (17, 66)])
 def test_int8_split_layout_and_digits(dev, ns, rows, K):
-    """The contiguous-row split (16-byte loads) and the generic one (same
-    data behind a one-block table) write identical planes; decoded from the
+    """One matrix stored four ways -- row-major (row kernel), row-major with
+    its column pairs permuted (row kernel through the column table),
+    column-major (column kernel), and row-major behind a one-block table
+    (generic kernel) -- gives identical planes; decoded from the
     fragment-major layout, the digits lie in [-128, 127] and reconstruct
     X = rint(x 2^(8ns-2-e)) with e the exponent of the row max (frexp)."""
     rng = np.random.default_rng(rows * 7 + K + ns)
@@ -151,8 +153,18 @@ def test_int8_split_layout_and_digits(dev, ns, rows, K):
     gen_view = ops.View(rows, K, np.arange(rows) * K, np.arange(K), blk=np.zeros(1, np.int64),
                         rpb=rows, cpb=K)
     gen = mixed.Split(mixed.I8, Ad, gen_view, ns)
+    # column pairs stored in reverse order: cofs[c] = (K/2 - 1 - c//2) * 2 + c % 2
+    cp = np.arange(K)
+    cofs = (K // 2 - 1 - cp // 2) * 2 + cp % 2
+    Ap = np.empty_like(A)
+    Ap[:, cofs] = A
+    perm = mixed.Split(mixed.I8, torch.from_numpy(Ap).to(dev),
+                       ops.View(rows, K, np.arange(rows) * K, cofs), ns)
+    At = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
+    colmaj = mixed.Split(mixed.I8, At, ops.View(rows, K, np.arange(rows), np.arange(K) * rows), ns)
     torch.cuda.synchronize()
-    assert torch.equal(fast.buf, gen.buf) and torch.equal(fast.rexp, gen.rexp)
+    for other in (gen, perm, colmaj):
+        assert torch.equal(fast.buf, other.buf) and torch.equal(fast.rexp, other.rexp)
     rp, kp = (rows + 127) // 128 * 128, (K + 127) // 128 * 128
     planes = fast.buf.cpu().numpy().view(np.int8).reshape(ns, rp // 16, kp // 64, 4, 16, 16)
     # [digit][row block][K step][kq][row % 16][byte] -> [digit][row][k]
