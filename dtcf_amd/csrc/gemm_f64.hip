@@ -316,9 +316,12 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, uint32_t q, int64_t
                 for (int j = 0; j < FN; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
             // The other stage was last read before the previous barrier, so
-            // the next step's tile can be stored halfway through this one's
-            // MFMAs; only the barrier is left for the end of the step.
-            if (s == 1 && more) {
+            // the next step's tile can be stored any time in this step: here
+            // after the last k-slice's MFMAs are issued, which gives its
+            // global loads the whole step.  (Measured on the ring step: this
+            // placement 63.4-63.6 TF/s, after slice 1 62.9, after the loop --
+            // outside the unrolled body -- 61.2.)
+            if (s == 3 && more) {
                 la.store(smem + (cur ^ 1) * STAGE, wave, lane);
                 lb.store(smem + (cur ^ 1) * STAGE + LDS_A, wave, lane);
             }
