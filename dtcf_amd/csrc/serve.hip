@@ -12,11 +12,12 @@
 // (256-byte granules) on the host.  Finalizing a program places every
 // temporary (and each mixed-precision contraction's slice scratch) at a
 // fixed offset of a request block by walking the statements in program
-// order; wherever a placement reuses bytes of a temporary that died earlier,
-// the statement gets an extra dependency on that temporary's last reader
-// (write-after-read), so any dependency-respecting issue order is safe and a
-// request never allocates after admission.  A request is admitted when its
-// block fits in the arena.
+// order, either each in bytes of its own or, when that block would be much
+// larger, reusing the bytes of temporaries that died earlier; wherever a
+// placement reuses bytes, the statement gets an extra dependency on every
+// reader of the previous occupant (write-after-read), so any
+// dependency-respecting issue order is safe and a request never allocates
+// after admission.  A request is admitted when its block fits in the arena.
 //
 // Issue.  A scheduler thread keeps `nslots` non-blocking streams.  Each
 // statement of an admitted request becomes ready when all its dependencies
@@ -234,6 +235,8 @@ struct dtcf_program_s {
     int64_t block = 0;
     double flops = 0;
     bool final = false;
+    int layout = DTCF_LAYOUT_AUTO;  // requested placement (DTCF_LAYOUT_*)
+    bool reuse = false;   // placement reuses freed temporaries (adds write-after-read edges)
     bool broken = false;  // a statement failed to plan: the program is unusable
     int refs = 0;  // live requests
 };
@@ -944,49 +947,81 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
         for (int a : ops)
             if (p->arrs[a].role != DTCF_ARR_INPUT) s.deps.push_back(p->arrs[a].producer);
     }
-    // Static placement in program order.  A freed region remembers every
-    // statement that touched its last occupant (all readers of a temporary,
-    // or its producer if nothing reads it; the owning statement for slice
-    // scratch): statements complete out of program order, so a placement
-    // that reuses the region waits for all of them, not only the last.
-    Arena plan;
-    plan.init(int64_t(1) << 60);
-    struct Dead {
-        int64_t off, len;
-        std::vector<int> waits;
-    };
-    std::vector<Dead> dead;
-    auto place = [&](int i, int64_t bytes) {
-        const int64_t off = plan.alloc(bytes), len = granule(bytes);
-        for (const auto &d : dead)
-            if (d.off < off + len && off < d.off + d.len)
-                for (int w : d.waits)
-                    if (w != i) p->st[i].deps.push_back(w);
-        return off;
-    };
+    // Static placement in program order, in one of two layouts.
+    //  * reuse: first fit, a temporary's bytes free once its last reader is
+    //    placed.  A freed region remembers every statement that touched its
+    //    last occupant (all readers of a temporary, or its producer if
+    //    nothing reads it; the owning statement for slice scratch):
+    //    statements complete out of program order, so a placement that
+    //    reuses the region waits for all of them, not only the last.
+    //  * separate: every temporary and scratch region its own bytes, so no
+    //    write-after-read edges -- only the data dependencies order the
+    //    statements.
+    // The separate layout is taken when its block is within 1.5x (or
+    // 256 MiB) of the reuse layout's: small programs then run as wide as
+    // their data dependencies allow, large ones keep the smaller block.
     std::vector<std::vector<int>> dies(ns);
     for (int a = 0; a < (int)p->arrs.size(); a++) {
         const PArr &x = p->arrs[a];
         if (x.role == DTCF_ARR_TEMP && x.producer >= 0)
             dies[x.last_use >= 0 ? x.last_use : x.producer].push_back(a);
     }
-    for (int i = 0; i < ns; i++) {
-        PStmt &s = p->st[i];
-        PArr &C = p->arrs[s.c];
-        if (C.role == DTCF_ARR_TEMP) C.off = place(i, C.elems * (int64_t)sizeof(double));
-        if (s.scratch_bytes > 0) {
-            s.scratch_off = place(i, s.scratch_bytes);
-            plan.release(s.scratch_off, s.scratch_bytes);
-            dead.push_back({s.scratch_off, granule(s.scratch_bytes), {i}});
+    struct Dead {
+        int64_t off, len;
+        std::vector<int> waits;
+    };
+    // Places everything (offsets into the arrays and statements); with
+    // reuse, collects the write-after-read edges into war.  Returns the
+    // block size.
+    auto layout = [&](bool reuse, std::vector<std::vector<int>> &war) -> int64_t {
+        Arena plan;
+        plan.init(int64_t(1) << 60);
+        std::vector<Dead> dead;
+        auto place = [&](int i, int64_t bytes) {
+            const int64_t off = plan.alloc(bytes), len = granule(bytes);
+            for (const auto &d : dead)
+                if (d.off < off + len && off < d.off + d.len)
+                    for (int w : d.waits)
+                        if (w != i) war[i].push_back(w);
+            return off;
+        };
+        for (int i = 0; i < ns; i++) {
+            PStmt &s = p->st[i];
+            PArr &C = p->arrs[s.c];
+            if (C.role == DTCF_ARR_TEMP) C.off = place(i, C.elems * (int64_t)sizeof(double));
+            if (s.scratch_bytes > 0) {
+                s.scratch_off = place(i, s.scratch_bytes);
+                if (reuse) {
+                    plan.release(s.scratch_off, s.scratch_bytes);
+                    dead.push_back({s.scratch_off, granule(s.scratch_bytes), {i}});
+                }
+            }
+            if (!reuse) continue;
+            for (int a : dies[i]) {
+                PArr &x = p->arrs[a];
+                plan.release(x.off, x.elems * (int64_t)sizeof(double));
+                dead.push_back({x.off, granule(x.elems * (int64_t)sizeof(double)),
+                                x.readers.empty() ? std::vector<int>{x.producer} : x.readers});
+            }
         }
-        for (int a : dies[i]) {
-            PArr &x = p->arrs[a];
-            plan.release(x.off, x.elems * (int64_t)sizeof(double));
-            dead.push_back({x.off, granule(x.elems * (int64_t)sizeof(double)),
-                            x.readers.empty() ? std::vector<int>{x.producer} : x.readers});
-        }
+        return std::max<int64_t>(plan.high, kGranule);
+    };
+    std::vector<std::vector<int>> war(ns), none(ns);
+    const int64_t sep_block = layout(false, none);
+    const int64_t reuse_block = layout(true, war);
+    const bool separate = p->layout == DTCF_LAYOUT_SEPARATE ||
+                          (p->layout == DTCF_LAYOUT_AUTO &&
+                           sep_block <= std::max(reuse_block + (int64_t(256) << 20),
+                                                 reuse_block + reuse_block / 2));
+    if (separate) {
+        p->block = layout(false, none);
+        p->reuse = false;
+    } else {
+        p->block = reuse_block;
+        p->reuse = true;
+        for (int i = 0; i < ns; i++)
+            p->st[i].deps.insert(p->st[i].deps.end(), war[i].begin(), war[i].end());
     }
-    p->block = std::max<int64_t>(plan.high, kGranule);
     p->flops = 0;
     for (int i = 0; i < ns; i++) {
         PStmt &s = p->st[i];
@@ -999,6 +1034,23 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
         p->flops += s.flops;
     }
     p->final = true;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_program_set_layout(dtcf_program p, int layout)
+{
+    DTCF_REQUIRE(p && !p->final, "program: finalized or null");
+    DTCF_REQUIRE(layout == DTCF_LAYOUT_AUTO || layout == DTCF_LAYOUT_REUSE ||
+                     layout == DTCF_LAYOUT_SEPARATE,
+                 "program: unknown layout");
+    p->layout = layout;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_program_get_layout(dtcf_program p, int *layout)
+{
+    DTCF_REQUIRE(p && p->final && layout, "program: not finalized or null");
+    *layout = p->reuse ? DTCF_LAYOUT_REUSE : DTCF_LAYOUT_SEPARATE;
     return DTCF_OK;
 }
 

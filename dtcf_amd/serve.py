@@ -47,6 +47,8 @@ def _declare():
         "dtcf_program_sum": [vp, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_char_p),
                              C.POINTER(C.c_char_p), C.POINTER(dbl)],
         "dtcf_program_finalize": [vp],
+        "dtcf_program_set_layout": [vp, C.c_int],
+        "dtcf_program_get_layout": [vp, C.POINTER(C.c_int)],
         "dtcf_program_info": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(dbl), C.POINTER(i64)],
         "dtcf_program_destroy": [vp],
         "dtcf_request_submit": [vp, vp, C.POINTER(vp), C.POINTER(i64)],
@@ -196,12 +198,16 @@ class System:
         return sum(int(np.prod(s)) * 8 for s in self.shape.values())
 
 
+LAYOUT_AUTO, LAYOUT_REUSE, LAYOUT_SEPARATE = 0, 1, 2
+
+
 class Program:
     """One term list planned over a system.  ``expose`` makes every assigned
-    array an output (for parity tests of each statement)."""
+    array an output (for parity tests of each statement); ``layout`` picks the
+    placement of temporaries (LAYOUT_*; ``self.layout`` is the one used)."""
 
     def __init__(self, server: Server, tl: termlist.TermList, system: System, prec=0, nslice=5,
-                 max_level=0, expose=False):
+                 max_level=0, expose=False, layout=LAYOUT_AUTO):
         self.server, self.tl, self.system = server, tl, system
         L = lib()
         h = C.c_void_p()
@@ -241,6 +247,7 @@ class Program:
                     idx = (C.c_char_p * nt)(*[",".join(t.idx).encode() for t in s.terms])
                     coefs = (C.c_double * nt)(*[t.coef for t in s.terms])
                     check(L.dtcf_program_sum(h, s.lhs.name.encode(), ci, nt, names, idx, coefs))
+            check(L.dtcf_program_set_layout(h, int(layout)))
             check(L.dtcf_program_finalize(h))
         except Exception:
             L.dtcf_program_destroy(h)
@@ -251,6 +258,9 @@ class Program:
         check(L.dtcf_program_info(h, C.byref(blk), C.byref(nst), C.byref(fl), C.byref(nout)))
         self.block_bytes, self.nstatements, self.flops = blk.value, nst.value, fl.value
         assert nout.value == len(self.outputs)
+        lay = C.c_int()
+        check(L.dtcf_program_get_layout(h, C.byref(lay)))
+        self.layout = lay.value
 
     def new_outputs(self, device="cuda"):
         return [torch.empty(int(np.prod(s)), dtype=torch.float64, device=device)

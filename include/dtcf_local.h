@@ -452,8 +452,17 @@ int dtcf_program_contract(dtcf_program p, const char *c, const char *c_idx, cons
 int dtcf_program_sum(dtcf_program p, const char *c, const char *c_idx, int nterms,
                      const char *const *names, const char *const *idx, const double *coefs);
 /* Plans every statement (operand views, sum strides), the dependency graph
- * and the static placement of temporaries inside one request block. */
+ * and the static placement of temporaries inside one request block: each
+ * temporary in bytes of its own (DTCF_LAYOUT_SEPARATE: only data
+ * dependencies order the statements) or first fit reusing dead temporaries'
+ * bytes with write-after-read dependencies (DTCF_LAYOUT_REUSE: the smallest
+ * block).  DTCF_LAYOUT_AUTO (default) takes SEPARATE when its block is within
+ * 1.5x (or 256 MiB) of REUSE's.  set_layout before finalize; get_layout
+ * after it reports the layout used. */
+enum { DTCF_LAYOUT_AUTO = 0, DTCF_LAYOUT_REUSE = 1, DTCF_LAYOUT_SEPARATE = 2 };
+int dtcf_program_set_layout(dtcf_program p, int layout);
 int dtcf_program_finalize(dtcf_program p);
+int dtcf_program_get_layout(dtcf_program p, int *layout);
 /* block_bytes: arena bytes one request reserves; flops: 2*m*n*k summed over
  * contractions plus one per summed element. */
 int dtcf_program_info(dtcf_program p, int64_t *block_bytes, int64_t *nstatements, double *flops,

@@ -56,17 +56,20 @@ def test_every_statement_matches_oracle(dev, lists, which):
         prog.destroy()
 
 
+@pytest.mark.parametrize("layout", [S.LAYOUT_REUSE, S.LAYOUT_SEPARATE])
 @pytest.mark.parametrize("which", ["t1", "t2"])
-def test_concurrent_program_matches_oracle(dev, lists, which):
-    """Temporaries share the request block; with 8 slots statements finish out
-    of program order, so a reused region must wait for every reader of its
-    previous occupant."""
+def test_concurrent_program_matches_oracle(dev, lists, which, layout):
+    """With 8 slots statements finish out of program order.  Reuse layout:
+    temporaries share bytes of the request block, so a reused region must
+    wait for every reader of its previous occupant.  Separate layout: only
+    the data dependencies order the statements."""
     tl = lists[which]
     sizes = {"Oa": 4, "Ob": 3, "Va": 12, "Vb": 10}
-    print(f"serve concurrent {which} sizes {sizes}", flush=True)
+    print(f"serve concurrent {which} sizes {sizes} layout {layout}", flush=True)
     with S.Server(1 << 30, nslots=8) as srv:
         sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
-        prog = S.Program(srv, tl, sysm)
+        prog = S.Program(srv, tl, sysm, layout=layout)
+        assert prog.layout == layout
         ref = oracle_run(tl, sysm)
         for rep in range(3):
             got = prog.run()
@@ -202,3 +205,19 @@ def test_task_timestamps_ordered(dev, lists):
         assert np.all(d <= inf.t_done)
         srv.release(rid)
         p.destroy()
+
+
+def test_auto_layout_choice(dev, lists):
+    """AUTO keeps temporaries apart when that costs little block memory and
+    reuses bytes otherwise; the separate block is never smaller."""
+    tl = lists["t2"]
+    with S.Server(8 << 30, nslots=4) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes={"Oa": 4, "Ob": 3, "Va": 12, "Vb": 10})
+        auto = S.Program(srv, tl, sysm)
+        reuse = S.Program(srv, tl, sysm, layout=S.LAYOUT_REUSE)
+        sep = S.Program(srv, tl, sysm, layout=S.LAYOUT_SEPARATE)
+        assert sep.block_bytes >= reuse.block_bytes
+        # a tiny system: the separate block is far below the 256 MiB slack
+        assert auto.layout == S.LAYOUT_SEPARATE and auto.block_bytes == sep.block_bytes
+        for p in (auto, reuse, sep):
+            p.destroy()
