@@ -64,13 +64,6 @@ void strides_of(const Space &S, const dtcf_dltc_tensor *t, int64_t *st)
     }
 }
 
-int64_t elems_of(const Space &S, const dtcf_dltc_tensor *t)
-{
-    int64_t n = 1;
-    for (int d = 0; d < t->dims; d++) n *= S.range(t->ids[d]);
-    return n;
-}
-
 // Tile-major offsets along a list of index names: entry r for
 // r = rank(tile digits over `ids`) * prod(wb) + rank(intra digits).
 std::vector<int64_t> tile_major_offsets(const Space &S, const std::vector<int> &ids,
