@@ -160,6 +160,64 @@ def leg_blocksparse(dev, log, steps=5):
     return out
 
 
+def leg_c1(dev, log, reps=20):
+    """C1 CCD ladder R[a,b,i,j] += 0.5 V[a,b,c,d] T[c,d,i,j] at o=8, v=32
+    (ccsd-t2.cc:57-61) through the DLTC term path: latency of the whole
+    term as one tile (wbO=8, wbV=32: one GEMM) and of the tile-task form
+    (wbO=4, wbV=8: one task per (external tile, summed tile), as DLTC's
+    finest granularity issues them), inputs antisymmetrised on the device."""
+    import itertools
+    import torch
+    from dtcf_amd import dltc, mixed
+    o, v = 8, 32
+    V = mixed.antisymmetrize4(mixed.fill_normal(v ** 4, 1, 0.05, dev), (v, v, v, v))
+    T = mixed.antisymmetrize4(mixed.fill_normal(v * v * o * o, 2, 0.05, dev), (v, v, o, o))
+    h = {f"h{i}": i - 1 for i in range(1, 13)}
+    p = {f"p{i}": 11 + i for i in range(1, 13)}
+    c_ids = [p["p3"], p["p4"], h["h1"], h["h2"]]
+    a_ids = [p["p5"], p["p6"], h["h1"], h["h2"]]
+    b_ids = [p["p3"], p["p4"], p["p5"], p["p6"]]
+    flops = 2.0 * v ** 4 * o ** 2
+    out = {"workload": "C1 CCD ladder o=8 v=32, FP64", "flops": flops}
+    R = torch.zeros(v * v * o * o, dtype=torch.float64, device=dev)
+    term = dltc.Term(dltc.make_space(o, v, 8, 32), c_ids, R, a_ids, T, b_ids, V, 0.5)
+    term.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        term.run()
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    out["one_tile_us"] = round(us, 2)
+    out["one_tile_tflops"] = round(flops / us / 1e6, 3)
+    wbO, wbV = 4, 8
+    term2 = dltc.Term(dltc.make_space(o, v, wbO, wbV), c_ids, R, a_ids, T, b_ids, V, 0.5)
+    ext, sumi = c_ids, [p["p5"], p["p6"]]
+    spO, spV = o // wbO, v // wbV
+    boxes = []
+    for e in itertools.product(range(spV), range(spV), range(spO), range(spO)):
+        for sm in itertools.product(range(spV), range(spV)):
+            lb, ub = [0] * 24, [0] * 24
+            for x, val in zip(ext + sumi, e + sm):
+                lb[x], ub[x] = val, val + 1
+            boxes.append((lb, ub))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for lb, ub in boxes:
+        term2.task(lb, ub)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    out["tasks"] = len(boxes)
+    out["tasks_ms"] = round(dt * 1e3, 3)
+    out["us_per_task"] = round(dt * 1e6 / len(boxes), 2)
+    log(f"[c1] {json.dumps(out)}")
+    del V, T, R
+    torch.cuda.empty_cache()
+    return out
+
+
 def leg_serving(log, requests=36, rate=2.5):
     """Served CCSD iteration stream: UHF T1 + T2 programs over o/v in
     {(32,128), (64,256), (96,384)}, Poisson arrivals, 200 GB HBM budget."""
@@ -276,7 +334,8 @@ def main():
     torch.cuda.empty_cache()
     extra = {}
     if not args.no_extra:
-        for key, fn in (("precision", lambda: leg_precision(log)),
+        for key, fn in (("c1", lambda: leg_c1(dev, log)),
+                        ("precision", lambda: leg_precision(log)),
                         ("blocksparse", lambda: leg_blocksparse(dev, log)),
                         ("serving", lambda: leg_serving(log))):
             try:
