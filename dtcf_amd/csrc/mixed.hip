@@ -2,8 +2,9 @@
 // synthetic inputs.
 //
 // An FP64 operand (a dtcf_view, rows x K) is split once into low-precision
-// slices laid out [slice][rows_padded][K_padded] (K contiguous, padded with
-// zeros to multiples of 128 so the GEMM needs no bounds checks):
+// slices of rows_padded x K_padded elements (zeros to multiples of 128, so
+// the GEMM needs no bounds checks), K contiguous -- for INT8 digits in
+// fragment-major order (i8_frag_offset):
 //   F32   : one FP32 slice                              (FP32 MFMA)
 //   BF16x3: x = s1 + s2 + s3 + r, s_i = bf16(rest)      (BF16 MFMA, FP32 accum)
 //   F16x3 : the same in FP16 after scaling each row by a power of two so its
@@ -362,8 +363,9 @@ __global__ __launch_bounds__(1024) void split_i8_cols_kernel(const double *__res
 // ------------------------------------------------------------------ GEMM
 // Tiles 128 x 128 x (128 bytes of K), 256 threads, 4 waves of 64 x 64 as
 // 4 x 4 MFMA 16x16 accumulators.  Operand slices are [rows_p][K_p] with K
-// contiguous.  LDS is fragment-major: [row/16][k step][lane][VB bytes],
-// lane = row%16 + 16*kq, each wave fragment read one contiguous span.
+// contiguous (INT8 digit planes fragment-major, read as such).  LDS is
+// fragment-major: [row/16][k step][lane][VB bytes], lane = row%16 + 16*kq,
+// each wave fragment read one contiguous span.
 template <int PREC>
 struct LP;
 template <>
@@ -565,11 +567,13 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
 // P = min(L-1, NS) products, so INT32 stays exact for P * 128^2 * Kc < 2^31;
 // every Kc bytes of K (a multiple of 64) the levels are flushed into C.
 //
-// Workgroup: C tile 128 x 128 (wave layout below).  LDS: two stages of 16 * NS fragments of
-// 1 KiB ([operand digit][16-row block][lane][16 B], the fragment-major order
-// of the per-product kernel: lane = row % 16 + 16 * kq holds bytes
-// 16kq..16kq+15 of the 64-byte step of its row).  Each fragment is one
-// direct global->LDS load of 64 lanes x 16 B, issued a step ahead.
+// Workgroup: C tile 128 x 128 (wave layout below).  LDS holds K steps as
+// fragments of 1 KiB ([operand digit][16-row block][lane][16 B], the order
+// of the digit planes themselves: lane = row % 16 + 16 * kq holds bytes
+// 16kq..16kq+15 of the 64-byte step of its row); each fragment is one
+// direct global->LDS load of 64 lanes x 16 B of one contiguous KiB.
+// `ozaki_kernel` keeps two full stages (16 * NS fragments each, loaded a
+// step ahead); `ozaki_ring_kernel` (NS <= 4) five half stages.
 constexpr int OZ_BM = 128, OZ_BN = 128, OZ_BK = 64;
 
 struct OzArgs {
@@ -585,12 +589,12 @@ struct OzArgs {
 };
 
 // Workgroup of 8 waves (2 per SIMD) as 4 x 2, each owning 32 x 64 of the
-// 128 x 128 tile as 2 x 4 MFMA blocks; the two LDS stages take all 160 KiB
-// at NS = 5.  The tile is as large as the registers allow (L - 1 INT32
-// level sums per C element) because the kernel's limit is L2 reuse: every
-// workgroup streams 16 * NS KiB of digit planes per K step, and a 64 x 128
-// tile (1.5x the L2 traffic per product) ran 15-25% slower, two 64 x 64
-// workgroups per CU slower still.
+// 128 x 128 tile as 2 x 4 MFMA blocks; the LDS stages take all 160 KiB (two
+// stages at NS = 5, five half stages at NS = 4).  The tile is as large as
+// the registers allow (L - 1 INT32 level sums per C element): every
+// workgroup streams 16 * NS KiB of digit planes per K step, the loads'
+// latency is the kernel's limit, and a 64 x 128 tile (1.5x the traffic per
+// product) ran 15-25% slower, two 64 x 64 workgroups per CU slower still.
 constexpr int OZ_WAVES = 8, OZ_WM = 32, OZ_WN = 64;
 constexpr int OZ_FM = OZ_WM / 16, OZ_FN = OZ_WN / 16;  // MFMA blocks per wave
 
