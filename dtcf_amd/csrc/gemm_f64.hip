@@ -303,8 +303,16 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, uint32_t q, int64_t
         }
         const double *a_s = smem + cur * STAGE + gA * 256 + lane;
         const double *b_s = smem + cur * STAGE + LDS_A + gB * 256 + lane;
+        // View operands: the wave starting a K step's MFMAs takes issue
+        // priority for its first k-slice, so the two workgroups sharing a
+        // CU drift apart and one's loads, stores and barrier overlap the
+        // other's MFMAs (ring step 63.5 -> 66.1 TF/s; priority over the
+        // whole step, or for the dense modes, measured slower).
+        if constexpr (VIEWS) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
+            if constexpr (VIEWS)
+                if (s == 1) __builtin_amdgcn_s_setprio(0);
             double af[FM], bf[FN];
 #pragma unroll
             for (int i = 0; i < FM; i++) af[i] = a_s[i * 256 + s * 64];
