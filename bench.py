@@ -32,8 +32,9 @@ sys.path.insert(0, ROOT)
 METRIC = "contraction TFLOP/s FP64 vs mixed at stated error; task p50/p99 latency"
 FP64_PEAK_TF = 78.6      # MI355X dense FP64 MFMA
 HBM_PEAK_GBS = 8000.0
-# measured HBM bytes per GEMM launch, keyed (mode, o, v)
-TRAFFIC_BYTES = {("fused", 48, 192): 19.91e9}
+# measured HBM bytes per GEMM launch, keyed (mode, o, v): rocprofv3 FETCH_SIZE x2 +
+# WRITE_SIZE x1 (factors from a plain 2 GiB copy in the same run)
+TRAFFIC_BYTES = {("fused", 48, 192): 36.04e9}
 
 
 def sig(x, n=4):
