@@ -649,8 +649,14 @@ __device__ __forceinline__ void oz_step(i4 (&acc)[L - 1][OZ_FM][OZ_FN], const i4
     i4 bcur[OZ_FN], bnxt[OZ_FN];
 #pragma unroll
     for (int j = 0; j < OZ_FN; j++) bcur[j] = sb[(wn * (OZ_WN / 16) + j) * 64 + lane];
+    // Issue priority for the first B digit's MFMAs: the SIMD's two waves
+    // drift apart, so one's fragment reads and barrier wait overlap the
+    // other's MFMAs (1.8% on the int8x4/L6 ladder; priority while issuing
+    // the loads instead gained nothing).
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int dj = 1; dj <= NS; dj++) {
+        if (dj == 2) __builtin_amdgcn_s_setprio(0);
         if (dj < NS) {
 #pragma unroll
             for (int j = 0; j < OZ_FN; j++)

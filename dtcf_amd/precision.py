@@ -55,8 +55,13 @@ class System:
         raw = mixed.fill_normal(v * v * o * o, seed + 3, 0.05, dev)
         self.Vvvoo = mixed.antisymmetrize4(raw, (v, v, o, o))
         raw = mixed.fill_normal(o * o * v * v, seed + 4, 0.05, dev)
-        self.W = mixed.antisymmetrize4(raw, (o, o, v, v))  # [k,j,c,b]
+        W = mixed.antisymmetrize4(raw, (o, o, v, v))  # W[k,j,c,b]
         del raw
+        # The ring's integral is kept as Wr[b,j,c,k] = W[k,j,c,b]: integrals
+        # do not change between iterations, so they are stored once in the
+        # layout the ring GEMM reads (rows (b,j), K (c,k), K contiguous).
+        self.Wr = ops.rrr_transpose(W, [3, 1, 2, 0], [o, o, v, v])
+        del W
         # V_oovv[i,j,a,b] = V_vvoo[a,b,i,j]
         self.Voovv = ops.rrr_transpose(self.Vvvoo, [2, 3, 0, 1], [v, v, o, o])
         self.T = ops.denom_update(self.Vvvoo, self.eo, self.ev)       # [c,d,i,j]
@@ -64,21 +69,19 @@ class System:
         # ladder GEMM: C (ab x ij) += A (ab x cd) * B (ij x cd)^T
         self.lad_a = ops.View(vv, vv, np.arange(vv) * vv, np.arange(vv))
         self.lad_b = ops.View(oo, vv, np.arange(oo), np.arange(vv) * oo)
-        # ring GEMM over T'[i,k,a,c] (T stored [a,c,i,k] read as [i,k,a,c])
-        # rows (i,a), cols (c,k) with k fastest, so T's rows are K-contiguous;
-        # W rows (j,b) with b contiguous, cols (c,k); C rows (i,a), cols (j,b)
+        # ring GEMM R[a,b,i,j] += sum_{k,c} T[a,c,i,k] W[k,j,c,b]:
+        # rows (a,i), cols (b,j), K (c,k), the inner index fastest in each, so
+        # T's rows are K-contiguous in runs of o, Wr's rows are K-contiguous,
+        # and a C tile covers whole o x o blocks of R (contiguous writes).
         #   T[a,c,i,k] at ((a*v + c)*o + i)*o + k
-        ia = np.array([i * o + a * v * o * o for i in range(o) for a in range(v)])
-        kc = np.array([k + c * o * o for c in range(v) for k in range(o)])
-        jb = np.array([j * v * v + b for j in range(o) for b in range(v)])
-        kcw = np.array([k * o * v * v + c * v for c in range(v) for k in range(o)])
+        ai = np.array([a * v * o * o + i * o for a in range(v) for i in range(o)])
+        ck = np.array([c * o * o + k for c in range(v) for k in range(o)])
         #   R[a,b,i,j] at ((a*v + b)*o + i)*o + j
-        ia_r = np.array([a * v * o * o + i * o for i in range(o) for a in range(v)])
-        jb_r = np.array([b * o * o + j for j in range(o) for b in range(v)])
+        bj_r = np.array([b * o * o + j for b in range(v) for j in range(o)])
         n = o * v
-        self.ring_a = ops.View(n, n, ia, kc)
-        self.ring_b = ops.View(n, n, jb, kcw)
-        self.ring_c = ops.View(n, n, ia_r, jb_r)
+        self.ring_a = ops.View(n, n, ai, ck)
+        self.ring_b = ops.View(n, n, np.arange(n) * n, np.arange(n))   # Wr: dense [(b,j)][(c,k)]
+        self.ring_c = ops.View(n, n, ai, bj_r)
         self.lad_flops = 2.0 * vv * oo * vv
         self.ring_flops = 2.0 * n * n * n
         _sync()
@@ -104,7 +107,7 @@ class System:
                                                        self.lad_a, self.T, self.lad_b, R, None,
                                                        ldc=o * o))
             timed("gemm_ring", lambda: ops.gemm_view(o * v, o * v, o * v, 1.0, self.T, self.ring_a,
-                                                     self.W, self.ring_b, R, self.ring_c))
+                                                     self.Wr, self.ring_b, R, self.ring_c))
         else:
             # integrals (V_vvvv, W) are the same in every CC iteration: their
             # slices can stay resident; the amplitudes are split every time
@@ -114,7 +117,7 @@ class System:
                                                           max_level=level))
             del sa, sb
             ra = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.ring_a, nslice))
-            rb = timed("split_integrals", lambda: mixed.Split(prec, self.W, self.ring_b, nslice))
+            rb = timed("split_integrals", lambda: mixed.Split(prec, self.Wr, self.ring_b, nslice))
             timed("gemm_ring", lambda: mixed.gemm_split(ra, rb, R, alpha=1.0, cview=self.ring_c,
                                                         max_level=level))
             del ra, rb
