@@ -175,6 +175,22 @@ def test_gemm_group_f64_vs_oracle(dev, cfg):
             assert np.array_equal(got.reshape(m, ldc)[:, n:], ref.reshape(m, ldc)[:, n:])
 
 
+@pytest.mark.parametrize("m,n,k", [(2200, 2100, 520), (2049, 2300, 1040)])
+def test_gemm_one_partial_round_split(dev, m, n, k):
+    """More tiles than half the CU slots but at most all of them: every tile
+    is cut into K slices filling whole rounds (e.g. 306 tiles -> 3 slices);
+    accumulate into an existing C, edge tiles included, against numpy."""
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    C0 = rng.standard_normal((m, n))
+    Cd = torch.from_numpy(C0.copy()).to(dev)
+    ops.gemm(0, 1, m, n, k, 0.5, torch.from_numpy(A).to(dev), k, torch.from_numpy(B).to(dev), k,
+             Cd, n)
+    ref = C0 + 0.5 * (A @ B.T)
+    assert relerr(Cd.cpu().numpy(), ref) <= GEMM_TOL
+
+
 def test_gemm_repeat_is_bitwise_deterministic(dev):
     m = n = k = 384
     A = torch.from_numpy(RNG.standard_normal((m, k))).to(dev)
