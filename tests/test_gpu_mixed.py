@@ -102,11 +102,13 @@ def test_device_generators_match_host(dev):
 
 @pytest.mark.parametrize("ns,lev", [(5, 6), (4, 5), (4, 6), (5, 5), (3, 4), (4, 4)])
 @pytest.mark.parametrize("m,n,k,cview", [(200, 136, 520, False), (64, 300, 96, True),
-                                         (130, 260, 27000, False), (70, 130, 27000, True)])
+                                         (130, 260, 27000, False), (70, 130, 27000, True),
+                                         (130, 70, 70000, False), (48, 96, 70000, True)])
 def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
-    """The fused kernel sums each level exactly in INT32 (flushing every
-    26,624 K at most) and adds the levels to C once; the per-product path adds
-    every product to C.  Same digits, so the results agree to FP64 rounding."""
+    """The fused kernels sum each level exactly in INT32 and add the levels to
+    C at most every 26,176 (five digits) or 32,704 (four) K -- K = 70,000
+    takes two flushes inside the loop -- while the per-product path adds every
+    product to C.  Same digits, so the results agree to FP64 rounding."""
     rng = np.random.default_rng(m * n + k + ns)
     A = rng.standard_normal((m, k))
     B = rng.standard_normal((n, k))
