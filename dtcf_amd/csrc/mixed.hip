@@ -1097,9 +1097,18 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
         o.ldc = ldc;
         o.tiles_m = (uint32_t)(g.Mp / OZ_BM);
         o.tiles_n = (uint32_t)(g.Np / OZ_BN);
-        // INT32 exact while (products per level) * 128^2 * K < 2^31
-        const int per_level = std::min(lev - 1, ns);
-        const int64_t kmax = ((int64_t(1) << 31) - 1) / ((int64_t)per_level * 128 * 128);
+        // Level l = i + j sums the products q_i q_j, |q_1| <= 65, |q_i| <= 128
+        // otherwise; INT32 stays exact while K times the largest level's
+        // bound is below 2^31 (four digits to level 6: 49,408 per K, a flush
+        // every 43,456 K; five: 65,792, every 32,640 K).
+        int64_t worst = 0;
+        for (int l = 2; l <= lev; l++) {
+            int64_t s = 0;
+            for (int i = std::max(1, l - ns); i <= std::min(ns, l - 1); i++)
+                s += (i == 1 ? 65 : 128) * (l - i == 1 ? 65 : 128);
+            worst = std::max(worst, s);
+        }
+        const int64_t kmax = ((int64_t(1) << 31) - 1) / worst;
         o.kchunk_steps = std::max<int64_t>(1, kmax / OZ_BK);
         bool done = false;
         int st = oz_dispatch(ns, lev, o, cv != nullptr, s, &done);
