@@ -278,6 +278,10 @@ struct PackWork {
     int32_t op, r0;
 };
 
+struct MoveWork {
+    int32_t op, r0, c0;  // c0: the tile's first column (transposed moves only)
+};
+
 __global__ __launch_bounds__(256) void pack_kernel(const BOp *ops, const BOp *pops,
                                                    const PackWork *work, int64_t nwork,
                                                    const double *__restrict__ src,
@@ -300,33 +304,132 @@ __global__ __launch_bounds__(256) void pack_kernel(const BOp *ops, const BOp *po
 // Block moves between block-sparse storage and dense matrices: move m takes
 // block ops[m] (read through its descriptors) to dense[place.base + r * pitch
 // + c] (gather), or adds alpha times that region into the block (scatter;
-// blocks are disjoint, so no two moves write one element).  Work item = 8
-// rows of one move; threads run along the row.
+// blocks are disjoint, so no two moves write one element).  The dense side
+// is contiguous along c.  A block whose stride-1 digit lies on its row axis
+// instead (place.trans) moves as 32 x 32 tiles transposed through LDS, so
+// that both sides are read and written along their contiguous runs (lanes
+// along c there read one element per cache line: 0.8-1.2 TB/s).  The others
+// run lanes along c in chunks 2^cwsh columns wide (64, 128 or 256, whichever
+// pads the block's columns least), a work item being a band of
+// 8 x 256 / 2^cwsh rows over all the columns, eight loads per thread.
 struct Place {
     int64_t base, pitch;
+    int32_t trans, cwsh;
 };
 
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void blockmove_kernel(const BOp *ops, const Place *places,
-                                                        const PackWork *work, int64_t nwork,
+                                                        const MoveWork *work, int64_t nwork,
                                                         double alpha, const double *__restrict__ in,
                                                         double *__restrict__ out)
 {
+    __shared__ double t[32][33];  // [c - c0][r - r0]
+    const int lo = threadIdx.x & 31, hi = threadIdx.x >> 5;
     for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-        const PackWork pw = work[w];
+        const MoveWork pw = work[w];
         const BOp &o = ops[pw.op];
         const Place pl = places[pw.op];
-        const int r1 = (int)min<int64_t>(pw.r0 + 8, o.rows);
-        for (int r = pw.r0; r < r1; r++) {
-            const int64_t roff = o.base + axis_off(o.r, (uint32_t)r);
-            const int64_t doff = pl.base + (int64_t)r * pl.pitch;
-            for (int64_t c = threadIdx.x; c < o.cols; c += 256) {
-                const int64_t so = roff + axis_off(o.c, (uint32_t)c);
-                if constexpr (SCATTER) out[so] += alpha * in[doff + c];
-                else out[doff + c] = in[so];
+        if (!pl.trans) {  // lanes along c on both sides
+            const int cw = 1 << pl.cwsh, rr = threadIdx.x >> pl.cwsh, rp = 256 >> pl.cwsh;
+            int64_t roff[8];
+            bool ok[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int r = pw.r0 + rr + rp * i;
+                ok[i] = r < o.rows;
+                roff[i] = ok[i] ? o.base + axis_off(o.r, (uint32_t)r) : 0;
             }
+            for (int c = (threadIdx.x & (cw - 1)); c < o.cols; c += cw) {
+                const int64_t coff = axis_off(o.c, (uint32_t)c);
+                double v[8], u[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {  // every load issued before the first store
+                    const int64_t dd = pl.base + (int64_t)(pw.r0 + rr + rp * i) * pl.pitch + c;
+                    if constexpr (SCATTER) {
+                        v[i] = ok[i] ? in[dd] : 0.0;
+                        u[i] = ok[i] ? out[roff[i] + coff] : 0.0;
+                    } else {
+                        v[i] = ok[i] ? in[roff[i] + coff] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    if (!ok[i]) break;
+                    const int64_t dd = pl.base + (int64_t)(pw.r0 + rr + rp * i) * pl.pitch + c;
+                    if constexpr (SCATTER) out[roff[i] + coff] = u[i] + alpha * v[i];
+                    else out[dd] = v[i];
+                }
+            }
+            continue;
+        }
+        // one 32 x 32 tile through LDS: the block side with lanes along r, the
+        // dense side with lanes along c (a work item per tile: a band of tiles
+        // per item, one barrier pair each, ran the B gathers 1.2-1.8x slower)
+        const int rl = pw.r0 + lo;
+        const int64_t roff = rl < o.rows ? o.base + axis_off(o.r, (uint32_t)rl) : 0;
+        {
+            const int c0 = pw.c0;
+            if constexpr (!SCATTER) {
+                if (rl < o.rows) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int c = c0 + hi + 8 * i;
+                        if (c < o.cols) t[hi + 8 * i][lo] = in[roff + axis_off(o.c, (uint32_t)c)];
+                    }
+                }
+                __syncthreads();
+                const int c = c0 + lo;
+                if (c < o.cols) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = pw.r0 + hi + 8 * i;
+                        if (r < o.rows) out[pl.base + (int64_t)r * pl.pitch + c] = t[lo][hi + 8 * i];
+                    }
+                }
+            } else {
+                const int c = c0 + lo;
+                if (c < o.cols) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = pw.r0 + hi + 8 * i;
+                        if (r < o.rows) t[lo][hi + 8 * i] = in[pl.base + (int64_t)r * pl.pitch + c];
+                    }
+                }
+                __syncthreads();
+                if (rl < o.rows) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int c2 = c0 + hi + 8 * i;
+                        if (c2 < o.cols)
+                            out[roff + axis_off(o.c, (uint32_t)c2)] += alpha * t[hi + 8 * i][lo];
+                    }
+                }
+            }
+            __syncthreads();  // the tile is read before the next one is written
         }
     }
+}
+
+// Digits i-1 and i of an axis merge when stride[i-1] == ext[i] * stride[i]
+// (the pair is one run): fewer divisions per element, same addresses.
+dtcf_axis merge_digits(const dtcf_axis &a)
+{
+    dtcf_axis m = a;
+    m.nd = 1;
+    m.ext[0] = a.ext[0];
+    m.stride[0] = a.stride[0];
+    for (int i = 1; i < a.nd; i++) {
+        if (m.stride[m.nd - 1] == a.ext[i] * a.stride[i] &&
+            (int64_t)m.ext[m.nd - 1] * a.ext[i] < (int64_t(1) << 31)) {
+            m.ext[m.nd - 1] *= a.ext[i];
+            m.stride[m.nd - 1] = a.stride[i];
+        } else {
+            m.ext[m.nd] = a.ext[i];
+            m.stride[m.nd] = a.stride[i];
+            m.nd++;
+        }
+    }
+    return m;
 }
 
 int64_t axis_max(const dtcf_axis &a)
@@ -571,7 +674,7 @@ extern "C" int dtcf_grouped_run(dtcf_grouped g, double alpha, const double *A, c
 struct dtcf_blockmove_s {
     BOp *ops = nullptr;
     Place *places = nullptr;
-    PackWork *work = nullptr;
+    MoveWork *work = nullptr;
     int64_t n = 0, nwork = 0, dense_span = 0;
 };
 
@@ -592,7 +695,7 @@ extern "C" int dtcf_blockmove_plan(const dtcf_block_op *ops, const int64_t *dens
     DTCF_REQUIRE(n >= 0 && (n == 0 || (ops && dense_base && dense_pitch)), "blockmove: bad lists");
     std::vector<BOp> hops(n);
     std::vector<Place> hpl(n);
-    std::vector<PackWork> hw;
+    std::vector<MoveWork> hw;
     int64_t span = 0;
     for (int64_t i = 0; i < n; i++) {
         const dtcf_block_op &o = ops[i];
@@ -602,15 +705,30 @@ extern "C" int dtcf_blockmove_plan(const dtcf_block_op *ops, const int64_t *dens
         for (int d = 0; d < o.cols.nd; d++) DTCF_REQUIRE(o.cols.ext[d] >= 1, "blockmove: extent < 1");
         DTCF_REQUIRE(o.base >= 0 && dense_base[i] >= 0, "blockmove: negative base");
         hops[i].base = o.base;
-        hops[i].r = make_axis(o.rows);
-        hops[i].c = make_axis(o.cols);
+        const dtcf_axis ra = merge_digits(o.rows), ca = merge_digits(o.cols);
+        hops[i].r = make_axis(ra);
+        hops[i].c = make_axis(ca);
         hops[i].rows = axis_len(o.rows);
         hops[i].cols = axis_len(o.cols);
         DTCF_REQUIRE(hops[i].rows < (1ll << 31) && hops[i].cols < (1ll << 31), "blockmove: block too long");
         DTCF_REQUIRE(dense_pitch[i] >= hops[i].cols, "blockmove: pitch shorter than the block's columns");
-        hpl[i] = Place{dense_base[i], dense_pitch[i]};
+        // transposed tiles where the block runs along its rows, not its columns
+        const bool cfast = ca.stride[ca.nd - 1] == 1, rfast = ra.stride[ra.nd - 1] == 1;
+        const bool trans = !cfast && rfast;
+        int cwsh = 8;  // widest tile whose column padding is within 10%, else the least padded
+        for (int sh = 8; sh >= 6; sh--) {
+            const int64_t w = int64_t(1) << sh, pad = (hops[i].cols + w - 1) / w * w;
+            const int64_t best = (hops[i].cols + (int64_t(1) << cwsh) - 1) >> cwsh << cwsh;
+            if (pad * 10 <= hops[i].cols * 11) { cwsh = sh; break; }
+            if (pad < best) cwsh = sh;
+        }
+        hpl[i] = Place{dense_base[i], dense_pitch[i], trans ? 1 : 0, cwsh};
         span = std::max(span, dense_base[i] + (hops[i].rows - 1) * dense_pitch[i] + hops[i].cols);
-        for (int64_t r = 0; r < hops[i].rows; r += 8) hw.push_back(PackWork{(int32_t)i, (int32_t)r});
+        // work items: 32 x 32 tiles (transposed), else bands of 8 x 256 / 2^cwsh rows
+        const int64_t tr = trans ? 32 : 8 * (256 >> cwsh), tc = trans ? 32 : hops[i].cols;
+        for (int64_t r = 0; r < hops[i].rows; r += tr)
+            for (int64_t c = 0; c < hops[i].cols; c += tc)
+                hw.push_back(MoveWork{(int32_t)i, (int32_t)r, (int32_t)c});
     }
     auto *b = new dtcf_blockmove_s();
     b->n = n;
@@ -619,11 +737,11 @@ extern "C" int dtcf_blockmove_plan(const dtcf_block_op *ops, const int64_t *dens
     hipError_t e = hipSuccess;
     if (n) e = hipMalloc(&b->ops, n * sizeof(BOp));
     if (e == hipSuccess && n) e = hipMalloc(&b->places, n * sizeof(Place));
-    if (e == hipSuccess && b->nwork) e = hipMalloc(&b->work, b->nwork * sizeof(PackWork));
+    if (e == hipSuccess && b->nwork) e = hipMalloc(&b->work, b->nwork * sizeof(MoveWork));
     if (e == hipSuccess && n) e = hipMemcpy(b->ops, hops.data(), n * sizeof(BOp), hipMemcpyHostToDevice);
     if (e == hipSuccess && n) e = hipMemcpy(b->places, hpl.data(), n * sizeof(Place), hipMemcpyHostToDevice);
     if (e == hipSuccess && b->nwork)
-        e = hipMemcpy(b->work, hw.data(), b->nwork * sizeof(PackWork), hipMemcpyHostToDevice);
+        e = hipMemcpy(b->work, hw.data(), b->nwork * sizeof(MoveWork), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         dtcf_blockmove_destroy(b);
         return hip_fail(e, "blockmove: plan upload");
