@@ -34,6 +34,7 @@ VARIANTS = [
     ("fp16x3", mixed.F16X3, 3, 0),
     ("int8x4", mixed.I8, 4, 0),      # 10 products, fused
     ("int8x4/L6", mixed.I8, 4, 6),   # 13 products, fused
+    ("int8x4/L6, ring L5", mixed.I8, 4, (6, 5)),  # ladder 13 products, ring 10
     ("int8x5/L5", mixed.I8, 5, 5),   # 10 products, fused
     ("int8x5", mixed.I8, 5, 0),      # 15 products, fused
     ("int8x6", mixed.I8, 6, 0),      # 21 products, one GEMM each
@@ -89,8 +90,10 @@ class System:
     def iterate(self, prec, nslice, level):
         """Returns (R, T_new, E, seconds per phase).  Phases are timed with a
         device synchronisation on each side, so they add up to the
-        contraction time."""
+        contraction time.  level: the largest digit level kept (0: nslice + 1),
+        or a pair (ladder, ring)."""
         o, v = self.o, self.v
+        lev_lad, lev_ring = level if isinstance(level, tuple) else (level, level)
         R = self.Vvvoo.clone()
         ph = {}
 
@@ -114,12 +117,12 @@ class System:
             sa = timed("split_integrals", lambda: mixed.Split(prec, self.Vvvvv, self.lad_a, nslice))
             sb = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.lad_b, nslice))
             timed("gemm_ladder", lambda: mixed.gemm_split(sa, sb, R, alpha=0.5, ldc=o * o,
-                                                          max_level=level))
+                                                          max_level=lev_lad))
             del sa, sb
             ra = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.ring_a, nslice))
             rb = timed("split_integrals", lambda: mixed.Split(prec, self.Wr, self.ring_b, nslice))
             timed("gemm_ring", lambda: mixed.gemm_split(ra, rb, R, alpha=1.0, cview=self.ring_c,
-                                                        max_level=level))
+                                                        max_level=lev_ring))
             del ra, rb
         Tn = ops.denom_update(R, self.eo, self.ev)
         E = ops.energy(self.Voovv, Tn, o, v).item()
@@ -153,7 +156,8 @@ def study(o, v, variants=VARIANTS, reps=1, log=print):
             rec.update(max_abs_err=d, max_rel_err=d / R64.abs().max().item(),
                        energy_err=abs(E - E64), speedup=t64 / tc,
                        speedup_integrals_resident=t64 / tcached,
-                       products=mixed.n_products(prec, ns, lev))
+                       products=(mixed.n_products(prec, ns, lev) if not isinstance(lev, tuple)
+                                 else [mixed.n_products(prec, ns, x) for x in lev]))
         out["variants"].append(rec)
         log(json.dumps(rec))
         del R, Tn
