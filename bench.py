@@ -14,7 +14,7 @@ FP32 / INT8 digit splits, ladder + ring at o=64, v=256), the block-sparse D2h
 terms as grouped launches, and the served UHF CCSD stream (throughput and
 p50 / p99 latency).  --no-extra skips them.
 
-Prints ONE JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line.  --gpus N > 1 exits: one rank's share runs on one GPU.
 """
 from __future__ import annotations
 
@@ -263,8 +263,8 @@ def main():
                     help="also run a plain 2 GiB device copy (for PMC traffic scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs one rank's share on one GPU; --gpus {args.gpus} is not "
-              "supported (see DESIGN.md for how the step shards across GPUs)", file=sys.stderr)
+        print(f"bench.py: one rank's share runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the inter-rank SUMMA and redistribution are outside this library)", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 

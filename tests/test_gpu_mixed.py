@@ -13,7 +13,7 @@ def relerr(got, ref):
     return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
 
-# normwise bound per mode at K ~ 500, data N(0,1); stated in DESIGN.md
+# normwise bound per mode at K ~ 500, data N(0,1)
 BOUNDS = {mixed.F32: 5e-6, mixed.BF16X3: 5e-6, mixed.F16X3: 5e-6}
 
 
