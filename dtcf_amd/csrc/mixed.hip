@@ -3,8 +3,8 @@
 //
 // An FP64 operand (a dtcf_view, rows x K) is split once into low-precision
 // slices of rows_padded x K_padded elements (zeros to multiples of 128, so
-// the GEMM needs no bounds checks), K contiguous -- for INT8 digits in
-// fragment-major order (i8_frag_offset):
+// the GEMM needs no bounds checks), K contiguous -- for BF16, FP16 and INT8
+// in fragment-major order over the bytes of a row (i8_frag_offset):
 //   F32   : one FP32 slice                              (FP32 MFMA)
 //   BF16x3: x = s1 + s2 + s3 + r, s_i = bf16(rest)      (BF16 MFMA, FP32 accum)
 //   F16x3 : the same in FP16 after scaling each row by a power of two so its
@@ -167,7 +167,9 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
     const size_t plane = (size_t)rows_p * K_p;
     for (int64_t c = lane; c < K_p; c += 64) {
         const double x = (real && c < K) ? src[vaddr(v, row, c)] : 0.0;
-        const size_t o = (size_t)row * K_p + c;
+        // F32 row-major; BF16 / F16 fragment-major over the bytes of a row
+        const size_t o = MODE == DTCF_PREC_F32 ? (size_t)row * K_p + c
+                                               : (size_t)(i8_frag_offset(row, 2 * c, 2 * K_p) >> 1);
         if constexpr (MODE == DTCF_PREC_F32) {
             ((float *)dst)[o] = (float)x;
         } else if constexpr (MODE == DTCF_PREC_BF16X3) {
@@ -448,16 +450,16 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
     // Global 16-byte chunks: pass i (0..3), wave w, lane l ->
     // row = 16*g + l%16 with g = 2*w + i/2, byte 64*(i%2) + 16*(l/16) of the
     // tile row.  Each chunk holds 16/VB fragments' worth of lanes.
-    // INT8 planes are fragment-major (i8_frag_offset): the same chunk is
-    // lane l's 16 bytes of fragment (row block m0/16 + grp, 64-byte step
-    // 2 kt + i%2).
+    // INT8, BF16 and F16 planes are fragment-major over the bytes of a row
+    // (i8_frag_offset): the same chunk is lane l's 16 bytes of fragment (row
+    // block m0/16 + grp, 64-byte step 2 kt + i%2).
     uint4 ra[4], rb[4];
     auto gload = [&](int64_t kt) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int grp = 2 * wave + (i >> 1);
-            if constexpr (PREC == DTCF_PREC_I8) {
-                const int64_t nk64 = g.Kp >> 6;
+            if constexpr (VB == 16) {  // INT8, BF16, F16: fragment-major planes
+                const int64_t nk64 = kb >> 6;
                 const int64_t fa = (((m0 >> 4) + grp) * nk64 + 2 * kt + (i & 1)) * 1024 + 16 * lane;
                 const int64_t fb = (((n0 >> 4) + grp) * nk64 + 2 * kt + (i & 1)) * 1024 + 16 * lane;
                 ra[i] = *(const uint4 *)((const unsigned char *)g.A + fa);

@@ -362,11 +362,12 @@ enum {
 int64_t dtcf_split_bytes(int prec, int nslice, int64_t rows, int64_t K);
 /* Split an FP64 operand given as a view (rows x K) into slices of
  * round_up(rows,128) x round_up(K,128) elements each, zero padded: row-major
- * (K contiguous) for F32 / BF16X3 / F16X3; for I8 fragment-major, the
- * 16 rows x 64 bytes of row block rb and K step kt forming one KiB at
- * ((rb * K_p/64 + kt) * 64 + r + 16 * (k%64 / 16)) * 16 + k%16 (the MFMA
- * operand order).  rexp[round_up(rows,128)] receives the per-row
- * power-of-two exponents for F16X3 and I8. */
+ * (K contiguous) for F32; for BF16X3 / F16X3 / I8 fragment-major over the
+ * bytes of a row (Kb = K_p times the element size), the 16 rows x 64 bytes
+ * of row block rb and 64-byte K step kt forming one KiB at
+ * ((rb * Kb/64 + kt) * 64 + r + 16 * (b%64 / 16)) * 16 + b%16 for byte b of
+ * row r (the MFMA operand order).  rexp[round_up(rows,128)] receives the
+ * per-row power-of-two exponents for F16X3 and I8. */
 int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view v, void *dst, int32_t *rexp,
                    void *stream);
 /* C (FP64) += alpha * sum over slice pairs (i, j), i + j <= max_level
