@@ -600,16 +600,50 @@ struct OzArgs {
 constexpr int OZ_WAVES = 8, OZ_WM = 32, OZ_WN = 64;
 constexpr int OZ_FM = OZ_WM / 16, OZ_FN = OZ_WN / 16;  // MFMA blocks per wave
 
-template <int NS, int L, bool CVIEW>
-__device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][OZ_FM][OZ_FN], int64_t m0,
-                                         int64_t n0, int wm, int wn, int lane)
+// Element traits of the fused kernels: the MFMA over one 16-byte fragment
+// per lane (64 bytes of K of 16 rows) and its accumulator.  A level's sum is
+// scaled back by 2^(ea + eb + 4 - 8 l) (INT8), 2^(ea + eb) (F16, rows scaled
+// by a power of two) or not at all (BF16).
+template <int P>
+struct OzT;
+template <>
+struct OzT<DTCF_PREC_I8> {
+    typedef i4 Acc;
+    __device__ static inline Acc mfma(i4 a, i4 b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct OzT<DTCF_PREC_BF16X3> {
+    typedef f4 Acc;
+    __device__ static inline Acc mfma(i4 a, i4 b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                       __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+    }
+};
+template <>
+struct OzT<DTCF_PREC_F16X3> {
+    typedef f4 Acc;
+    __device__ static inline Acc mfma(i4 a, i4 b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                      __builtin_bit_cast(h8, b), c, 0, 0, 0);
+    }
+};
+
+template <int P, int NS, int L, bool CVIEW>
+__device__ __forceinline__ void oz_flush(const OzArgs &g,
+                                         typename OzT<P>::Acc (&acc)[L - 1][OZ_FM][OZ_FN],
+                                         int64_t m0, int64_t n0, int wm, int wn, int lane)
 {
     const int fr = lane & 15, fq = lane >> 4;
 #pragma unroll
     for (int j = 0; j < OZ_FN; j++) {
         const int64_t col = n0 + wn * OZ_WN + j * 16 + fr;
         if (col >= g.N) continue;
-        const int ebc = g.eb[col];
+        const int ebc = P == DTCF_PREC_BF16X3 ? 0 : g.eb[col];
         int64_t cof;
         if constexpr (CVIEW) cof = g.cv.cofs[col];
         else cof = col;
@@ -619,11 +653,16 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][OZ_FM
             for (int r = 0; r < 4; r++) {
                 const int64_t row = m0 + wm * OZ_WM + i * 16 + 4 * fq + r;
                 if (row >= g.M) continue;
-                const int e = g.ea[row] + ebc;
+                const int e = P == DTCF_PREC_BF16X3 ? 0 : g.ea[row] + ebc;
                 double v = 0.0;
 #pragma unroll
-                for (int l = L - 2; l >= 0; l--)  // smallest terms first
-                    v += ldexp((double)acc[l][i][j][r], e + 4 - 8 * (l + 2));
+                for (int l = L - 2; l >= 0; l--) {  // smallest terms first
+                    if constexpr (P == DTCF_PREC_I8)
+                        v += ldexp((double)acc[l][i][j][r], e + 4 - 8 * (l + 2));
+                    else
+                        v += (double)acc[l][i][j][r];
+                }
+                if constexpr (P == DTCF_PREC_F16X3) v = ldexp(v, e);
                 int64_t addr;
                 if constexpr (CVIEW) addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + cof;
                 else addr = row * g.ldc + cof;
@@ -638,9 +677,9 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g, i4 (&acc)[L - 1][OZ_FM
 // while this one is used); two updates of one accumulator -- (di, dj) and
 // (di - 1, dj + 1) -- are about 8 (NS - 1) MFMAs apart, and the SIMD's
 // second wave interleaves its own chains.
-template <int NS, int L>
-__device__ __forceinline__ void oz_step(i4 (&acc)[L - 1][OZ_FM][OZ_FN], const i4 *sa, const i4 *sb,
-                                        int wm, int wn, int lane)
+template <int P, int NS, int L>
+__device__ __forceinline__ void oz_step(typename OzT<P>::Acc (&acc)[L - 1][OZ_FM][OZ_FN],
+                                        const i4 *sa, const i4 *sb, int wm, int wn, int lane)
 {
     i4 af[NS][OZ_FM];
 #pragma unroll
@@ -671,8 +710,8 @@ __device__ __forceinline__ void oz_step(i4 (&acc)[L - 1][OZ_FM][OZ_FN], const i4
             for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
                 for (int j = 0; j < OZ_FN; j++)
-                    acc[di + dj - 2][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                        af[di - 1][i], bcur[j], acc[di + dj - 2][i][j], 0, 0, 0);
+                    acc[di + dj - 2][i][j] =
+                        OzT<P>::mfma(af[di - 1][i], bcur[j], acc[di + dj - 2][i][j]);
         }
 #pragma unroll
         for (int j = 0; j < OZ_FN; j++) bcur[j] = bnxt[j];
@@ -746,14 +785,14 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
             for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
 
     auto step = [&](const i4 *st) __attribute__((always_inline)) {
-        oz_step<NS, L>(acc, st, st + FA * 64, wm, wn, lane);
+        oz_step<DTCF_PREC_I8, NS, L>(acc, st, st + FA * 64, wm, wn, lane);
     };
     int64_t since = 0;
     auto after = [&](bool more) __attribute__((always_inline)) {
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads have landed
         __syncthreads();                // ... and everyone's; stage reads done
         if (++since == g.kchunk_steps && more) {
-            oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+            oz_flush<DTCF_PREC_I8, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
 #pragma unroll
             for (int l = 0; l < L - 1; l++)
 #pragma unroll
@@ -777,7 +816,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
         step(stage1);
         after(kt + 2 < nk);
     }
-    oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+    oz_flush<DTCF_PREC_I8, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
 }
 
 // The same computation with a deeper load pipeline, for NS <= 4: LDS holds
@@ -790,7 +829,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
 // flight (vmcnt = NS): direct loads complete in issue order.  The loop is
 // unrolled by five so every slot is a fixed object and the compiler sees
 // that loads into one slot never alias reads of another.
-template <int NS, int L, bool CVIEW>
+template <int P, int NS, int L, bool CVIEW>
 __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
 {
     static_assert(OZ_BM == OZ_BN, "A and B halves have the same size");
@@ -823,13 +862,14 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         }
     };
 
-    i4 acc[L - 1][OZ_FM][OZ_FN];
+    typedef typename OzT<P>::Acc Acc;
+    Acc acc[L - 1][OZ_FM][OZ_FN];
 #pragma unroll
     for (int l = 0; l < L - 1; l++)
 #pragma unroll
         for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
-            for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+            for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (Acc){0, 0, 0, 0};
 
     // s_waitcnt with vmcnt = PH and no expcnt / lgkmcnt wait (GFX9 encoding:
     // vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[15:14])
@@ -848,13 +888,13 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         else __builtin_amdgcn_s_waitcnt(0);
         barrier();
         if (++since == g.kchunk_steps && k + 1 < nk) {
-            oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+            oz_flush<P, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
 #pragma unroll
             for (int l = 0; l < L - 1; l++)
 #pragma unroll
                 for (int i = 0; i < OZ_FM; i++)
 #pragma unroll
-                    for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (i4){0, 0, 0, 0};
+                    for (int j = 0; j < OZ_FN; j++) acc[l][i][j] = (Acc){0, 0, 0, 0};
             since = 0;
         }
     };
@@ -874,7 +914,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         if (k >= nk) break;                               \
         if (k + 1 < nk) issue(NB, g.B, pb, n0, k + 1);    \
         if (k + 2 < nk) issue(NA, g.A, pa, m0, k + 2);    \
-        oz_step<NS, L>(acc, SA, SB, wm, wn, lane);        \
+        oz_step<P, NS, L>(acc, SA, SB, wm, wn, lane);     \
         after(k);                                         \
     }
     for (int64_t kt = 0; kt < nk; kt += 5) {
@@ -885,7 +925,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         OZ_RING_STEP(h3, h4, h1, h2, 4)
     }
 #undef OZ_RING_STEP
-    oz_flush<NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+    oz_flush<P, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
 }
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
@@ -894,14 +934,15 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // specialisation exists; 0: one launch per product.
 int g_fused_i8 = 1;
 
-template <int NS, int L>
+template <int P, int NS, int L>
 int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
 {
     const unsigned grid = g.tiles_m * g.tiles_n;
     if constexpr (NS <= 4) {
-        if (cview) hipLaunchKernelGGL((ozaki_ring_kernel<NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
-        else hipLaunchKernelGGL((ozaki_ring_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+        if (cview) hipLaunchKernelGGL((ozaki_ring_kernel<P, NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
+        else hipLaunchKernelGGL((ozaki_ring_kernel<P, NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
     } else {
+        static_assert(NS > 4 && P == DTCF_PREC_I8, "two full stages: INT8 with five digits only");
         if (cview) hipLaunchKernelGGL((ozaki_kernel<NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
         else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
     }
@@ -909,18 +950,26 @@ int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
     return DTCF_OK;
 }
 
-// Runs the fused kernel when (ns, lev) has a specialisation; *done = false
-// otherwise.
-int oz_dispatch(int ns, int lev, const OzArgs &g, bool cview, hipStream_t s, bool *done)
+// Runs the fused kernel when (prec, ns, lev) has a specialisation; *done =
+// false otherwise.
+int oz_dispatch(int prec, int ns, int lev, const OzArgs &g, bool cview, hipStream_t s, bool *done)
 {
     *done = true;
+    if (prec == DTCF_PREC_BF16X3 && ns == 3 && lev == 4)
+        return oz_launch<DTCF_PREC_BF16X3, 3, 4>(g, cview, s);
+    if (prec == DTCF_PREC_F16X3 && ns == 3 && lev == 4)
+        return oz_launch<DTCF_PREC_F16X3, 3, 4>(g, cview, s);
+    if (prec != DTCF_PREC_I8) {
+        *done = false;
+        return DTCF_OK;
+    }
     switch (ns * 16 + lev) {
-    case 3 * 16 + 4: return oz_launch<3, 4>(g, cview, s);
-    case 4 * 16 + 4: return oz_launch<4, 4>(g, cview, s);
-    case 4 * 16 + 5: return oz_launch<4, 5>(g, cview, s);
-    case 4 * 16 + 6: return oz_launch<4, 6>(g, cview, s);
-    case 5 * 16 + 5: return oz_launch<5, 5>(g, cview, s);
-    case 5 * 16 + 6: return oz_launch<5, 6>(g, cview, s);
+    case 3 * 16 + 4: return oz_launch<DTCF_PREC_I8, 3, 4>(g, cview, s);
+    case 4 * 16 + 4: return oz_launch<DTCF_PREC_I8, 4, 4>(g, cview, s);
+    case 4 * 16 + 5: return oz_launch<DTCF_PREC_I8, 4, 5>(g, cview, s);
+    case 4 * 16 + 6: return oz_launch<DTCF_PREC_I8, 4, 6>(g, cview, s);
+    case 5 * 16 + 5: return oz_launch<DTCF_PREC_I8, 5, 5>(g, cview, s);
+    case 5 * 16 + 6: return oz_launch<DTCF_PREC_I8, 5, 6>(g, cview, s);
     default: *done = false; return DTCF_OK;
     }
 }
@@ -1082,16 +1131,16 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
     // Slice pairs (i, j), 1-based, with i + j <= max_level (default ns + 1).
     const int lev = max_level > 0 ? max_level : ns + 1;
     hipStream_t s = as_stream(stream);
-    if (prec == DTCF_PREC_I8 && g_fused_i8) {
+    if (prec != DTCF_PREC_F32 && g_fused_i8) {
         OzArgs o{};
         o.A = (const unsigned char *)A_slices;
         o.B = (const unsigned char *)B_slices;
-        o.ea = ea;
-        o.eb = eb;
+        o.ea = scaled ? ea : nullptr;
+        o.eb = scaled ? eb : nullptr;
         o.alpha = alpha;
         o.M = m;
         o.N = n;
-        o.Kp = g.Kp;
+        o.Kp = g.Kp * (int64_t)esz;  // bytes per slice row
         o.Mp = g.Mp;
         o.Np = g.Np;
         o.C = C;
@@ -1103,6 +1152,7 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
         // otherwise; INT32 stays exact while K times the largest level's
         // bound is below 2^31 (four digits to level 6: 49,408 per K, a flush
         // every 43,456 K; five: 65,792, every 32,640 K).
+        // (BF16 / F16 levels sum in FP32 and are added to C at the end only.)
         int64_t worst = 0;
         for (int l = 2; l <= lev; l++) {
             int64_t s = 0;
@@ -1111,9 +1161,9 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
             worst = std::max(worst, s);
         }
         const int64_t kmax = ((int64_t(1) << 31) - 1) / worst;
-        o.kchunk_steps = std::max<int64_t>(1, kmax / OZ_BK);
+        o.kchunk_steps = prec == DTCF_PREC_I8 ? std::max<int64_t>(1, kmax / OZ_BK) : INT64_MAX;
         bool done = false;
-        int st = oz_dispatch(ns, lev, o, cv != nullptr, s, &done);
+        int st = oz_dispatch(prec, ns, lev, o, cv != nullptr, s, &done);
         if (st != DTCF_OK || done) return st;
     }
     if (prec == DTCF_PREC_I8)

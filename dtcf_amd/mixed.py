@@ -43,7 +43,8 @@ def _p(t):
 
 
 def set_fused(on: bool):
-    """INT8 digit products in one fused kernel (default) or one GEMM each."""
+    """Slice products (INT8 digits; three BF16 / FP16 slices at the default
+    level 4) in one fused kernel (default) or one GEMM each."""
     check(lib().dtcf_gemm_split_set_fused(1 if on else 0))
 
 

@@ -375,12 +375,13 @@ int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view v, void *d
  * accumulators are converted to FP64 and rescaled in the epilogue.  A_slices
  * from dtcf_split_f64 over an m x k view, B_slices over an n x k view.  C is
  * a view (cv) or dense row-major with leading dimension ldc. */
-/* INT8 digit products: 1 (default) runs every product of a call in one
- * fused kernel (level sums in INT32, flushed to C every <= 32,640 K, a
- * bound that relies on the top digit of dtcf_split_f64 lying within +-65)
- * where a specialisation exists -- (nslice, max_level) in {(3,4), (4,4),
- * (4,5), (4,6), (5,5), (5,6)}; otherwise, or with 0, one GEMM per product
- * (K < 131,072). */
+/* Slice products: 1 (default) runs every product of a call in one fused
+ * kernel where a specialisation exists -- I8 with (nslice, max_level) in
+ * {(3,4), (4,4), (4,5), (4,6), (5,5), (5,6)} (level sums in INT32, flushed
+ * to C every <= 32,640 K, a bound that relies on the top digit of
+ * dtcf_split_f64 lying within +-65), BF16X3 / F16X3 at max_level 4 (the
+ * default; level sums in FP32, added to C once); otherwise, or with 0, one
+ * GEMM per product (I8: K < 131,072). */
 int dtcf_gemm_split_set_fused(int on);
 int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k, double alpha,
                         const void *A_slices, const int32_t *ea, const void *B_slices,

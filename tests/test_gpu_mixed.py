@@ -139,6 +139,43 @@ def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
     assert e <= {(5, 6): 5e-9, (4, 5): 5e-7, (4, 6): 5e-8, (5, 5): 1e-6, (3, 4): 3e-5, (4, 4): 3e-5}[(ns, lev)]
 
 
+@pytest.mark.parametrize("prec", [mixed.BF16X3, mixed.F16X3])
+@pytest.mark.parametrize("m,n,k,cview", [(200, 136, 520, False), (64, 300, 96, True),
+                                         (130, 260, 27000, False), (70, 130, 27000, True)])
+def test_fused_bf16_f16_match_per_product(dev, prec, m, n, k, cview):
+    """Three BF16 / FP16 slices: the fused kernel sums the six products of
+    levels 2..4 in three FP32 accumulators and adds them to C once, the
+    per-product path adds each product to C.  Same slices, FP32 sums in a
+    different order: they agree to FP32 accumulation rounding, and both meet
+    the mode's bound against FP64."""
+    rng = np.random.default_rng(m * n + k + prec)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    C0 = rng.standard_normal((m, n))
+    Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    sa = mixed.Split(prec, Ad, mixed.dense_view(m, k))
+    sb = mixed.Split(prec, Bd, mixed.dense_view(n, k))
+    cv = ops.View(m, n, np.arange(m), np.arange(n) * m) if cview else None
+    out = []
+    try:
+        for fused in (True, False):
+            mixed.set_fused(fused)
+            Cd = torch.from_numpy((C0.T if cview else C0).copy().ravel()).to(dev)
+            mixed.gemm_split(sa, sb, Cd, alpha=0.75, ldc=n, cview=cv)
+            r = Cd.cpu().numpy()
+            out.append(r.reshape(n, m).T if cview else r.reshape(m, n))
+    finally:
+        mixed.set_fused(True)
+    ref = C0 + 0.75 * A @ B.T
+    d = relerr(out[0] - C0, out[1] - C0)
+    e = [relerr(o - C0, ref - C0) for o in out]
+    print(f"{mixed.NAMES[prec]} {m}x{n}x{k} cview={cview}: fused vs per-product {d:.1e}, "
+          f"vs FP64 {e[0]:.1e} / {e[1]:.1e}")
+    # FP32 sums over K (sqrt(K) growth): 5e-6 at K ~ 500, 3e-5 at K = 27,000
+    bound = 5e-6 if k <= 1000 else 3e-5
+    assert d <= bound and e[0] <= bound and e[1] <= bound
+
+
 @pytest.mark.parametrize("ns,lev", [(5, 6), (4, 5), (4, 6), (5, 5), (3, 4), (4, 4)])
 def test_fused_int8_flush_interval_at_largest_digits(dev, ns, lev):
     """Every element x = -(63 * 256^(ns-1) + 128 * (256^(ns-1) - 1) / 255) /
