@@ -133,7 +133,8 @@ def leg_precision(log):
     modes that bracket the 1e-7 Eh target."""
     from dtcf_amd import mixed, precision
     variants = [v for v in precision.VARIANTS
-                if v[0] in ("fp64", "fp32", "int8x4", "int8x4/L6", "int8x4/L6, ring L5", "int8x5")]
+                if v[0] in ("fp64", "fp32", "bf16x3", "int8x4", "int8x4/L6", "int8x4/L6, ring L5",
+                            "int8x5")]
     res = precision.study(64, 256, variants=variants, reps=2, log=lambda s_: log("[precision] " + s_))
     keep = ("name", "seconds", "seconds_integrals_resident", "tflops", "energy_err", "max_rel_err",
             "speedup", "speedup_integrals_resident", "products")
