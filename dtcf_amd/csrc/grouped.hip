@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const BOp *ops, const BOp *po
 // + c] (gather), or adds alpha times that region into the block (scatter;
 // blocks are disjoint, so no two moves write one element).  The dense side
 // is contiguous along c.  A block whose stride-1 digit lies on its row axis
-// instead (place.trans) moves as 32 x 32 tiles transposed through LDS, so
+// instead (place.trans) moves as 32 x 64 tiles transposed through LDS, so
 // that both sides are read and written along their contiguous runs (lanes
 // along c there read one element per cache line: 0.8-1.2 TB/s).  The others
 // run lanes along c in chunks 2^cwsh columns wide (64, 128 or 256, whichever
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void blockmove_kernel(const BOp *ops, const Pl
                                                         double alpha, const double *__restrict__ in,
                                                         double *__restrict__ out)
 {
-    __shared__ double t[32][33];  // [c - c0][r - r0]
+    __shared__ double t[64][33];  // [c - c0][r - r0]
     const int lo = threadIdx.x & 31, hi = threadIdx.x >> 5;
     for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
         const MoveWork pw = work[w];
@@ -362,51 +362,50 @@ __global__ __launch_bounds__(256) void blockmove_kernel(const BOp *ops, const Pl
             }
             continue;
         }
-        // one 32 x 32 tile through LDS: the block side with lanes along r, the
-        // dense side with lanes along c (a work item per tile: a band of tiles
-        // per item, one barrier pair each, ran the B gathers 1.2-1.8x slower)
-        const int rl = pw.r0 + lo;
+        // one 32 x 64 tile (rows x columns) through LDS: the block side with
+        // 32 lanes along r, the dense side with 64 lanes along c (a work item
+        // per tile; a band of tiles per item, one barrier pair each, ran the
+        // B gathers 1.2-1.8x slower)
+        const int rl = pw.r0 + lo, cl = threadIdx.x & 63, rq = threadIdx.x >> 6;
         const int64_t roff = rl < o.rows ? o.base + axis_off(o.r, (uint32_t)rl) : 0;
-        {
-            const int c0 = pw.c0;
-            if constexpr (!SCATTER) {
-                if (rl < o.rows) {
+        const int c0 = pw.c0;
+        if constexpr (!SCATTER) {
+            if (rl < o.rows) {
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int c = c0 + hi + 8 * i;
-                        if (c < o.cols) t[hi + 8 * i][lo] = in[roff + axis_off(o.c, (uint32_t)c)];
-                    }
-                }
-                __syncthreads();
-                const int c = c0 + lo;
-                if (c < o.cols) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int r = pw.r0 + hi + 8 * i;
-                        if (r < o.rows) out[pl.base + (int64_t)r * pl.pitch + c] = t[lo][hi + 8 * i];
-                    }
-                }
-            } else {
-                const int c = c0 + lo;
-                if (c < o.cols) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int r = pw.r0 + hi + 8 * i;
-                        if (r < o.rows) t[lo][hi + 8 * i] = in[pl.base + (int64_t)r * pl.pitch + c];
-                    }
-                }
-                __syncthreads();
-                if (rl < o.rows) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int c2 = c0 + hi + 8 * i;
-                        if (c2 < o.cols)
-                            out[roff + axis_off(o.c, (uint32_t)c2)] += alpha * t[hi + 8 * i][lo];
-                    }
+                for (int i = 0; i < 8; i++) {
+                    const int c = c0 + hi + 8 * i;
+                    if (c < o.cols) t[hi + 8 * i][lo] = in[roff + axis_off(o.c, (uint32_t)c)];
                 }
             }
-            __syncthreads();  // the tile is read before the next one is written
+            __syncthreads();
+            const int c = c0 + cl;
+            if (c < o.cols) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int r = pw.r0 + rq + 4 * i;
+                    if (r < o.rows) out[pl.base + (int64_t)r * pl.pitch + c] = t[cl][rq + 4 * i];
+                }
+            }
+        } else {
+            const int c = c0 + cl;
+            if (c < o.cols) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int r = pw.r0 + rq + 4 * i;
+                    if (r < o.rows) t[cl][rq + 4 * i] = in[pl.base + (int64_t)r * pl.pitch + c];
+                }
+            }
+            __syncthreads();
+            if (rl < o.rows) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int c2 = c0 + hi + 8 * i;
+                    if (c2 < o.cols)
+                        out[roff + axis_off(o.c, (uint32_t)c2)] += alpha * t[hi + 8 * i][lo];
+                }
+            }
         }
+        __syncthreads();  // the tile is read before the next one is written
     }
 }
 
@@ -724,8 +723,8 @@ extern "C" int dtcf_blockmove_plan(const dtcf_block_op *ops, const int64_t *dens
         }
         hpl[i] = Place{dense_base[i], dense_pitch[i], trans ? 1 : 0, cwsh};
         span = std::max(span, dense_base[i] + (hops[i].rows - 1) * dense_pitch[i] + hops[i].cols);
-        // work items: 32 x 32 tiles (transposed), else bands of 8 x 256 / 2^cwsh rows
-        const int64_t tr = trans ? 32 : 8 * (256 >> cwsh), tc = trans ? 32 : hops[i].cols;
+        // work items: 32 x 64 tiles (transposed), else bands of 8 x 256 / 2^cwsh rows
+        const int64_t tr = trans ? 32 : 8 * (256 >> cwsh), tc = trans ? 64 : hops[i].cols;
         for (int64_t r = 0; r < hops[i].rows; r += tr)
             for (int64_t c = 0; c < hops[i].cols; c += tc)
                 hw.push_back(MoveWork{(int32_t)i, (int32_t)r, (int32_t)c});
