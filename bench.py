@@ -215,6 +215,26 @@ def leg_c1(dev, log, reps=20):
     out["tasks"] = len(boxes)
     out["tasks_ms"] = round(dt * 1e3, 3)
     out["us_per_task"] = round(dt * 1e6 / len(boxes), 2)
+
+    # the same tasks recorded once as a HIP graph (the 64 output tiles'
+    # chains of 16 sum-tile tasks over 4 branches) and replayed
+    per = len(boxes) // (spV * spV * spO * spO)
+
+    def chain(i):
+        return lambda: [term2.task(lb, ub) for lb, ub in boxes[i:i + per]]
+
+    graph = dltc.TaskGraph([chain(i) for i in range(0, len(boxes), per)], branches=4)
+    graph.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / reps
+    out["tasks_graph_ms"] = round(gms, 3)
+    out["us_per_task_graph"] = round(gms * 1e3 / len(boxes), 2)
+    del graph
     log(f"[c1] {json.dumps(out)}")
     del V, T, R
     torch.cuda.empty_cache()

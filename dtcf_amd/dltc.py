@@ -116,6 +116,45 @@ class Term:
             pass
 
 
+class TaskGraph:
+    """Tile tasks recorded once as a HIP graph and replayed as one launch.
+
+    DLTC issues one task per tile box (DLTC/task.cc:83-137, driven by the
+    loops of DLTC/ccsd-t2.cc); a task is a handful of small GEMMs, each a
+    kernel of a few microseconds, and issued one by one from the host they
+    cost about 10 us each.  `chains` is a list of callables, each enqueueing
+    on the current stream tasks that must run in order (the sum tiles of one
+    output tile); different chains must write disjoint outputs.  Under
+    capture the chains are spread over `branches` streams forked from and
+    joined back into the capturing stream, so the graph has that many
+    independent branches; every replay() re-runs the recorded kernels with
+    the same buffers, which must stay allocated and in place.  Per output
+    tile the order of the tasks is the eager one, so results are bitwise
+    those of issuing the chains one after the other.  Call each entry point
+    once eagerly before capturing (first calls set kernel attributes)."""
+
+    def __init__(self, chains, branches=4):
+        import torch
+        self.graph = torch.cuda.CUDAGraph()
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, branches))]
+        with torch.cuda.graph(self.graph):
+            cur = torch.cuda.current_stream()
+            fork = torch.cuda.Event()
+            fork.record(cur)
+            for st in self.streams:
+                st.wait_event(fork)
+            for i, chain in enumerate(chains):
+                with torch.cuda.stream(self.streams[i % len(self.streams)]):
+                    chain()
+            for st in self.streams:
+                join = torch.cuda.Event()
+                join.record(st)
+                cur.wait_event(join)
+
+    def replay(self):
+        self.graph.replay()
+
+
 def task_sum(space, c_ids, C_t, a_ids, A_t, coef=1.0, itlb=None, itub=None):
     dc, da = desc(ids_of(c_ids), C_t), desc(ids_of(a_ids), A_t)
     lb = None if itlb is None else (C.c_int32 * NIDX)(*itlb)

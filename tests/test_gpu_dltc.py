@@ -120,6 +120,58 @@ def test_ccd_ladder_config1(dev, wbO, wbV):
     say("tasks", ntask)
 
 
+def test_task_graph_replays_match_eager(dev):
+    """The 1024 tile tasks of the o=8, v=32 ladder (wbO=4, wbV=8) recorded as
+    one HIP graph, the 64 output tiles' chains of 16 sum-tile tasks spread
+    over 1 and 4 branches: every replay -- with eager kernels and
+    synchronisations between replays -- gives bitwise the result of issuing
+    the tasks one by one (same kernels, same order per output tile)."""
+    import itertools
+    o, v, wbO, wbV = 8, 32, 4, 8
+    V, T = _ladder_inputs(o, v)
+    h = {f"h{i}": i - 1 for i in range(1, 13)}
+    p = {f"p{i}": 11 + i for i in range(1, 13)}
+    c_ids = [p["p3"], p["p4"], h["h1"], h["h2"]]
+    a_ids = [p["p5"], p["p6"], h["h1"], h["h2"]]
+    b_ids = [p["p3"], p["p4"], p["p5"], p["p6"]]
+    sp = dltc.make_space(o, v, wbO, wbV)
+    Td, Vd = dev_t(T, dev), dev_t(V, dev)
+    R0 = torch.from_numpy(np.random.default_rng(8).standard_normal(v * v * o * o)).to(dev)
+    spO, spV = o // wbO, v // wbV
+    chains = []
+    for e in itertools.product(range(spV), range(spV), range(spO), range(spO)):
+        boxes = []
+        for sm in itertools.product(range(spV), range(spV)):
+            lb, ub = [0] * 24, [0] * 24
+            for x, val in zip(c_ids + [p["p5"], p["p6"]], e + sm):
+                lb[x], ub[x] = val, val + 1
+            boxes.append((lb, ub))
+        chains.append(boxes)
+    Re = R0.clone()
+    eager = dltc.Term(sp, c_ids, Re, a_ids, Td, b_ids, Vd, 0.5)
+    for boxes in chains:
+        for lb, ub in boxes:
+            eager.task(lb, ub)
+    torch.cuda.synchronize()
+    ref = Re.cpu()
+    Rg = R0.clone()
+    term = dltc.Term(sp, c_ids, Rg, a_ids, Td, b_ids, Vd, 0.5)
+
+    def chain(boxes):
+        return lambda: [term.task(lb, ub) for lb, ub in boxes]
+
+    for branches in (1, 4):
+        g = dltc.TaskGraph([chain(b) for b in chains], branches=branches)
+        for rep in range(3):
+            Rg.copy_(R0)
+            eager.run()                      # eager work between replays
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            say("graph replay", branches, rep)
+            assert torch.equal(Rg.cpu(), ref)
+
+
 def test_task_mul_matches_tile_restatement(dev):
     # an uneven ext-tile box vs the oracle's tile-by-tile task_mul
     O, V, wbO, wbV = 6, 8, 2, 4
