@@ -559,15 +559,17 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
         }
 }
 
-// ------------------------------------------------------ fused INT8 digits
-// All digit products of one C tile in one kernel.  Per K step of 64 bytes
-// the workgroup stages every digit plane of its A rows and B rows in LDS
+// ---------------------------------------------- fused slice products
+// All slice products of one C tile in one kernel.  Per K step of 64 bytes
+// the workgroup stages every slice plane of its A rows and B rows in LDS
 // once and runs every product (i, j), i + j <= L, from registers.  Products
-// of one level l = i + j share the scale 2^(ea + eb + 4 - 8l), so they sum
-// exactly in one INT32 accumulator per level: L-1 accumulator sets instead
-// of one FP64 read-modify-write of C per product.  A level holds at most
-// P = min(L-1, NS) products, so INT32 stays exact for P * 128^2 * Kc < 2^31;
-// every Kc bytes of K (a multiple of 64) the levels are flushed into C.
+// of one level l = i + j share one scale (INT8 digits: 2^(ea + eb + 4 - 8l)),
+// so they sum in one accumulator per level: L-1 accumulator sets instead
+// of one FP64 read-modify-write of C per product.  INT8 levels sum exactly
+// in INT32 and are flushed into C every Kc bytes of K (a multiple of 64)
+// sized from the level's digit bound (|q_1| <= 65, |q_i| <= 128); BF16 and
+// FP16 slices (64 bytes = 32 elements per step, the same fragment bytes and
+// lane order) sum in FP32 and reach C once.
 //
 // Workgroup: C tile 128 x 128 (wave layout below).  LDS holds K steps as
 // fragments of 1 KiB ([operand digit][16-row block][lane][16 B], the order
@@ -579,10 +581,10 @@ __global__ __launch_bounds__(256, 2) void lp_gemm_kernel(LPArgs g)
 constexpr int OZ_BM = 128, OZ_BN = 128, OZ_BK = 64;
 
 struct OzArgs {
-    const unsigned char *A, *B;  // digit planes [NS][Mp][Kp], [NS][Np][Kp]
-    const int32_t *ea, *eb;
+    const unsigned char *A, *B;  // slice planes [NS][Mp][Kp bytes], [NS][Np][Kp bytes]
+    const int32_t *ea, *eb;      // row exponents (INT8, F16) or null (BF16)
     double alpha;
-    int64_t M, N, Kp, Mp, Np;
+    int64_t M, N, Kp, Mp, Np;    // Kp: bytes of K per slice row (a multiple of 64)
     int64_t kchunk_steps;        // K steps between INT32 flushes
     double *C;
     ViewArgs cv;
