@@ -1132,6 +1132,7 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
     const size_t pa = (size_t)g.Mp * g.Kp * esz, pb = (size_t)g.Np * g.Kp * esz;
     // Slice pairs (i, j), 1-based, with i + j <= max_level (default ns + 1).
     const int lev = max_level > 0 ? max_level : ns + 1;
+    if (lev < 2) return DTCF_OK;  // no pair has i + j <= 1: C is unchanged
     hipStream_t s = as_stream(stream);
     if (prec != DTCF_PREC_F32 && g_fused_i8) {
         OzArgs o{};

@@ -102,3 +102,21 @@ def test_gemm_group_rejects_bad_problems_without_gpu_calls():
     arr = (P * 1)(ok)
     assert L.dtcf_gemm_group_f64(C.cast(arr, C.c_void_p), 1, 1.0, 1, 9, None) == -1
     assert L.dtcf_gemm_group_f64(None, 0, 1.0, 1, 0, None) == 0
+
+
+def test_gemm_split_level_one_is_empty_without_gpu_calls():
+    """max_level 1 keeps no slice pair (i + j >= 2): C is left alone and no
+    work is enqueued (the flush interval would divide by a zero level
+    bound); bad precisions are refused first."""
+    from dtcf_amd import mixed
+    from dtcf_amd._lib import lib
+
+    L = lib()
+    fake = 256  # aligned, never dereferenced
+    for prec, ns in ((mixed.I8, 4), (mixed.BF16X3, 3), (mixed.F16X3, 3)):
+        st = L.dtcf_gemm_split_f64(prec, ns, 64, 64, 64, 1.0, fake, fake, fake, fake, 1, fake,
+                                   None, 64, None)
+        assert st == 0, L.dtcf_last_error()
+    st = L.dtcf_gemm_split_f64(9, 4, 64, 64, 64, 1.0, fake, fake, fake, fake, 0, fake, None, 64,
+                               None)
+    assert st == -1 and b"precision" in L.dtcf_last_error()
