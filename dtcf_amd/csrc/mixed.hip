@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
             __bf16 *d = (__bf16 *)dst;
             double r = x;
             for (int s = 0; s < 3; s++) {
-                const __bf16 q = (__bf16)(float)r;  // round to nearest (via fp32)
+                const __bf16 q = (__bf16)(float)r;  // nearest BF16 of r, ties to even
                 d[s * plane + o] = q;
                 r -= (double)(float)q;
             }
@@ -359,6 +359,100 @@ __global__ __launch_bounds__(1024) void split_i8_cols_kernel(const double *__res
 #pragma unroll
         for (int s = 0; s < 6; s++)
             if (s < nslice) out[s * plane16 + kt * 64] = make_uint4(wd[s][0], wd[s][1], wd[s][2], wd[s][3]);
+    }
+}
+
+// Wide operands (few rows, long K: the T1 statements' A operands are a few
+// hundred rows by millions of K) leave the row-parallel kernels above with
+// a few hundred waves.  There the split runs in two K-parallel launches:
+// one block per row reduces the row maximum to its exponent, then every
+// thread writes one 16-byte chunk of each slice plane (16 INT8, 8 BF16 /
+// FP16 or, row-major, 4 FP32 elements) -- the same values, bit for bit, as
+// the row-parallel kernels.
+template <int MODE>
+__global__ __launch_bounds__(512) void row_exp_kernel(const double *__restrict__ src, ViewArgs v,
+                                                      int64_t rows, int64_t K,
+                                                      int32_t *__restrict__ rexp)
+{
+    __shared__ double part[8];
+    const int64_t row = blockIdx.x;
+    double m = 0.0;
+    if (row < rows)
+        for (int64_t c = threadIdx.x; c < K; c += 512) m = fmax(m, fabs(src[vaddr(v, row, c)]));
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 8; w++) m = fmax(m, part[w]);
+        int e = 0;
+        if (m > 0.0) frexp(m, &e);  // m < 2^e
+        if (MODE == DTCF_PREC_F16X3) e -= 14;
+        rexp[row] = e;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void split_wide_kernel(const double *__restrict__ src, ViewArgs v,
+                                                         int64_t rows, int64_t K, int64_t rows_p,
+                                                         int64_t K_p, int nslice,
+                                                         unsigned char *__restrict__ dst,
+                                                         const int32_t *__restrict__ rexp)
+{
+    constexpr int ESZ = MODE == DTCF_PREC_F32 ? 4 : MODE == DTCF_PREC_I8 ? 1 : 2;
+    constexpr int PER = 16 / ESZ;  // elements per 16-byte chunk
+    const int64_t cpr = K_p / PER, total = rows_p * cpr;
+    const size_t plane = (size_t)rows_p * K_p * ESZ;  // bytes
+    for (int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x; u < total;
+         u += (int64_t)gridDim.x * 256) {
+        const int64_t row = u / cpr, c0 = (u - row * cpr) * PER;
+        const bool real = row < rows;
+        double x[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) x[j] = (real && c0 + j < K) ? src[vaddr(v, row, c0 + j)] : 0.0;
+        if constexpr (MODE == DTCF_PREC_F32) {
+            float4 f = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+            *(float4 *)(dst + ((size_t)row * K_p + c0) * 4) = f;
+        } else {
+            const size_t off = (size_t)i8_frag_offset(row, c0 * ESZ, K_p * ESZ);  // chunk start
+            uint32_t wd[6][4] = {};
+            if constexpr (MODE == DTCF_PREC_I8) {
+                const int sh = 8 * nslice - 2 - rexp[row];
+#pragma unroll
+                for (int j = 0; j < PER; j++) {
+                    int q[6] = {0, 0, 0, 0, 0, 0};
+                    i8_digits(x[j], sh, nslice, q);
+#pragma unroll
+                    for (int sl = 0; sl < 6; sl++)
+                        wd[sl][j >> 2] |= (uint32_t)(q[sl] & 255) << (8 * (j & 3));
+                }
+            } else {
+                const int e = MODE == DTCF_PREC_F16X3 ? rexp[row] : 0;
+#pragma unroll
+                for (int j = 0; j < PER; j++) {
+                    double r = MODE == DTCF_PREC_F16X3 ? ldexp(x[j], -e) : x[j];
+#pragma unroll
+                    for (int sl = 0; sl < 3; sl++) {
+                        uint16_t bits;
+                        if constexpr (MODE == DTCF_PREC_BF16X3) {
+                            const __bf16 q = (__bf16)(float)r;  // as split_kernel
+                            bits = __builtin_bit_cast(uint16_t, q);
+                            r -= (double)(float)q;
+                        } else {
+                            const _Float16 q = (_Float16)(float)r;
+                            bits = __builtin_bit_cast(uint16_t, q);
+                            r -= (double)(float)q;
+                        }
+                        wd[sl][j >> 1] |= (uint32_t)bits << (16 * (j & 1));
+                    }
+                }
+            }
+            const int ns = MODE == DTCF_PREC_I8 ? nslice : 3;
+#pragma unroll
+            for (int sl = 0; sl < 6; sl++)
+                if (sl < ns)
+                    *(uint4 *)(dst + sl * plane + off) =
+                        make_uint4(wd[sl][0], wd[sl][1], wd[sl][2], wd[sl][3]);
+        }
     }
 }
 
@@ -1054,6 +1148,42 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
     const int64_t rp = round_up(rows, 128), kp = round_up(K, 128);
     const unsigned grid = (unsigned)((rp + 3) / 4);
     hipStream_t s = as_stream(stream);
+    // Waves the row-parallel kernel would run: the INT8 column-pair kernel 4
+    // per 16 rows, the row-run kernel 16 per 16 rows, the generic one per row.
+    const bool i8rows = prec == DTCF_PREC_I8 && !v->blk && v->colpairs &&
+                        (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+    const int64_t row_waves = i8rows ? rp / 4 : rp;
+    if (row_waves < 2048 && kp >= 8 * rp) {  // wide: K-parallel launches
+        const int ns = prec == DTCF_PREC_I8 ? nslice : prec == DTCF_PREC_F32 ? 1 : 3;
+        const int64_t esz = prec == DTCF_PREC_F32 ? 4 : prec == DTCF_PREC_I8 ? 1 : 2;
+        const int64_t chunks = rp * kp * esz / 16;
+        const unsigned g2 = (unsigned)std::min<int64_t>((chunks + 255) / 256, 1 << 16);
+        unsigned char *d = (unsigned char *)dst;
+        switch (prec) {
+        case DTCF_PREC_F32:
+            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_F32>, dim3(g2), dim3(256), 0, s, src, a,
+                               rows, K, rp, kp, ns, d, rexp);
+            break;
+        case DTCF_PREC_BF16X3:
+            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_BF16X3>, dim3(g2), dim3(256), 0, s, src, a,
+                               rows, K, rp, kp, ns, d, rexp);
+            break;
+        case DTCF_PREC_F16X3:
+            hipLaunchKernelGGL(row_exp_kernel<DTCF_PREC_F16X3>, dim3((unsigned)rp), dim3(512), 0, s,
+                               src, a, rows, K, rexp);
+            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_F16X3>, dim3(g2), dim3(256), 0, s, src, a,
+                               rows, K, rp, kp, ns, d, rexp);
+            break;
+        default:
+            hipLaunchKernelGGL(row_exp_kernel<DTCF_PREC_I8>, dim3((unsigned)rp), dim3(512), 0, s, src,
+                               a, rows, K, rexp);
+            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_I8>, dim3(g2), dim3(256), 0, s, src, a, rows,
+                               K, rp, kp, ns, d, rexp);
+            break;
+        }
+        DTCF_HIP(hipGetLastError());
+        return DTCF_OK;
+    }
     switch (prec) {
     case DTCF_PREC_F32:
         hipLaunchKernelGGL(split_kernel<DTCF_PREC_F32>, dim3(grid), dim3(256), 0, s, src, a, rows, K,

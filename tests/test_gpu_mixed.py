@@ -210,15 +210,70 @@ def test_fused_int8_flush_interval_at_largest_digits(dev, ns, lev):
     assert abs(out[0][0, 0] - ref) <= drop + 1e-14 * ref
 
 
+@pytest.mark.parametrize("prec", [mixed.F32, mixed.BF16X3, mixed.F16X3])
+@pytest.mark.parametrize("rows,K", [(40, 20000), (300, 520)])
+def test_float_split_bitwise_vs_host(dev, prec, rows, K):
+    """FP32 / BF16x3 / FP16x3 slices decoded from their planes equal a host
+    emulation bit for bit: each slice is the FP64 remainder rounded once to
+    the nearest BF16 / FP16 (ties to even), FP16 rows first scaled by
+    2^-(e-14), e the exponent of the row max.  40 x 20000 takes the
+    K-parallel launches, 300 x 520 the row-parallel kernel."""
+    rng = np.random.default_rng(rows + K + prec)
+    A = rng.standard_normal((rows, K)) * np.exp2(rng.integers(-8, 8, size=(rows, 1)))
+    A[1] = 0.0
+    sp = mixed.Split(prec, torch.from_numpy(A).to(dev), mixed.dense_view(rows, K))
+    torch.cuda.synchronize()
+    rp, kp = (rows + 127) // 128 * 128, (K + 127) // 128 * 128
+    buf = sp.buf.cpu().numpy()
+    Ah = torch.zeros((rp, kp), dtype=torch.float64)
+    Ah[:rows, :K] = torch.from_numpy(A)
+    if prec == mixed.F32:
+        got = buf.view(np.uint32).reshape(rp, kp)
+        assert np.array_equal(got, Ah.float().numpy().view(np.uint32))
+        return
+    # [slice][row block][64-byte step][kq][row % 16][8 elements] -> [slice][row][k]
+    got = buf.view(np.uint16).reshape(3, rp // 16, kp // 32, 4, 16, 8)
+    got = got.transpose(0, 1, 4, 2, 3, 5).reshape(3, rp, kp)
+    r = Ah.clone()
+    if prec == mixed.F16X3:
+        m = np.max(np.abs(Ah.numpy()), axis=1)
+        e = np.where(m > 0, np.frexp(m)[1], 0) - 14
+        assert np.array_equal(sp.rexp.cpu().numpy(), e.astype(np.int32))
+        r = torch.from_numpy(np.ldexp(Ah.numpy(), -e[:, None]))
+    r = r.numpy()
+
+    def bf16_round(x):  # nearest value with 8 significant bits, ties to even
+        _, ex = np.frexp(x)
+        step = np.ldexp(1.0, ex - 8)
+        return np.where(x == 0, 0.0, np.rint(x / step) * step)
+
+    for sl in range(3):
+        if prec == mixed.BF16X3:
+            qd = bf16_round(r)
+            want = (qd.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+        else:
+            q16 = r.astype(np.float16)
+            qd = q16.astype(np.float64)
+            want = q16.view(np.uint16)
+        bad = np.argwhere(got[sl] != want)
+        if len(bad):
+            i, k = bad[0]
+            print(f"slice {sl}: {len(bad)} differ, first ({i}, {k}): got {got[sl][i, k]:#06x} "
+                  f"want {want[i, k]:#06x}, remainder {r[i, k]!r}", flush=True)
+        assert not len(bad), sl
+        r = r - qd
+
+
 @pytest.mark.parametrize("ns", [4, 5, 6])
-@pytest.mark.parametrize("rows,K", [(200, 522), (128, 256), (17, 66)])
+@pytest.mark.parametrize("rows,K", [(200, 522), (128, 256), (17, 66), (40, 20000)])
 def test_int8_split_layout_and_digits(dev, ns, rows, K):
     """One matrix stored four ways -- row-major (row kernel), row-major with
     its column pairs permuted (row kernel through the column table),
     column-major (column kernel), and row-major behind a one-block table
     (generic kernel) -- gives identical planes; decoded from the
     fragment-major layout, the digits lie in [-128, 127] and reconstruct
-    X = rint(x 2^(8ns-2-e)) with e the exponent of the row max (frexp)."""
+    X = rint(x 2^(8ns-2-e)) with e the exponent of the row max (frexp).
+    40 x 20000 is wide: all four take the K-parallel launches."""
     rng = np.random.default_rng(rows * 7 + K + ns)
     A = rng.standard_normal((rows, K)) * np.exp2(rng.integers(-20, 20, size=(rows, 1)))
     A[3 % rows] = 0.0  # an all-zero row: e = 0, all digits 0
