@@ -650,6 +650,10 @@ int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, hipStream_t s)
     return DTCF_ERR_INVALID;
 }
 
+int gemm_workspace(hipStream_t s, size_t need, double **ws) { return split_workspace(s, need, ws); }
+
+int gemm_device_cus(int *cus) { return device_cus(cus); }
+
 int gemm_reserve_workspace(hipStream_t s)
 {
     // nsplit * sk <= 2 * slots for every configuration, and slots <= 2 per

@@ -75,6 +75,10 @@ int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, hipStream_t s);
 // Allocates the tail-split workspace of stream `s` at the largest size any
 // GEMM on it can need, so later launches on `s` never allocate.
 int gemm_reserve_workspace(hipStream_t s);
+// The split workspace of stream `s` (grown to `need` bytes if smaller; work
+// on one stream shares it in stream order), and the device's CU count.
+int gemm_workspace(hipStream_t s, size_t need, double **ws);
+int gemm_device_cus(int *cus);
 
 // dtcf_gemm_view_f64 with C = alpha * A * B instead of C += (capi.hip).
 int gemm_view_overwrite(int64_t m, int64_t n, int64_t k, double alpha, const double *A,

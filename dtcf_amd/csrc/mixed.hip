@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gemm_f64.hpp"
 #include "view.hpp"
 
 namespace dtcf {
@@ -680,6 +681,12 @@ struct OzArgs {
     double alpha;
     int64_t M, N, Kp, Mp, Np;    // Kp: bytes of K per slice row (a multiple of 64)
     int64_t kchunk_steps;        // K steps between INT32 flushes
+    // K split (grids with few tiles): workgroup b computes tile b / sk over
+    // K steps [(b % sk) * kslice, + kslice) and leaves alpha * its sum in
+    // ws[b] (OZ_BM x OZ_BN); oz_split_reduce adds the slices to C in order.
+    uint32_t sk;
+    int64_t kslice;
+    double *ws;
     double *C;
     ViewArgs cv;
     int64_t ldc;
@@ -732,7 +739,8 @@ struct OzT<DTCF_PREC_F16X3> {
 template <int P, int NS, int L, bool CVIEW>
 __device__ __forceinline__ void oz_flush(const OzArgs &g,
                                          typename OzT<P>::Acc (&acc)[L - 1][OZ_FM][OZ_FN],
-                                         int64_t m0, int64_t n0, int wm, int wn, int lane)
+                                         int64_t m0, int64_t n0, int wm, int wn, int lane,
+                                         double *wsp, bool first)
 {
     const int fr = lane & 15, fq = lane >> 4;
 #pragma unroll
@@ -759,6 +767,11 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g,
                         v += (double)acc[l][i][j][r];
                 }
                 if constexpr (P == DTCF_PREC_F16X3) v = ldexp(v, e);
+                if (wsp) {  // K slice: this workgroup's own tile of the workspace
+                    double *w = wsp + (row - m0) * OZ_BN + (col - n0);
+                    *w = first ? g.alpha * v : *w + g.alpha * v;
+                    continue;
+                }
                 int64_t addr;
                 if constexpr (CVIEW) addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + cof;
                 else addr = row * g.ldc + cof;
@@ -819,8 +832,8 @@ __device__ __forceinline__ void oz_step(typename OzT<P>::Acc (&acc)[L - 1][OZ_FM
 __device__ __forceinline__ void oz_tile(const OzArgs &g, int64_t &m0, int64_t &n0)
 {
     const uint32_t T = g.tiles_m * g.tiles_n;
-    uint32_t q = blockIdx.x;
-    if ((T & 7u) == 0) q = (q & 7u) * (T >> 3) + (q >> 3);
+    uint32_t q = blockIdx.x / g.sk;
+    if (g.sk == 1 && (T & 7u) == 0) q = (q & 7u) * (T >> 3) + (q >> 3);
     const uint32_t width = 8 * g.tiles_n;
     const uint32_t first = (q / width) * 8;
     const uint32_t gsz = min(g.tiles_m - first, 8u);
@@ -883,12 +896,15 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
     auto step = [&](const i4 *st) __attribute__((always_inline)) {
         oz_step<DTCF_PREC_I8, NS, L>(acc, st, st + FA * 64, wm, wn, lane);
     };
+    double *const wsp = g.ws ? g.ws + (int64_t)blockIdx.x * OZ_BM * OZ_BN : nullptr;
+    bool first = true;
     int64_t since = 0;
     auto after = [&](bool more) __attribute__((always_inline)) {
         __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads have landed
         __syncthreads();                // ... and everyone's; stage reads done
         if (++since == g.kchunk_steps && more) {
-            oz_flush<DTCF_PREC_I8, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+            oz_flush<DTCF_PREC_I8, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane, wsp, first);
+            first = false;
 #pragma unroll
             for (int l = 0; l < L - 1; l++)
 #pragma unroll
@@ -899,20 +915,22 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
         }
     };
 
-    const int64_t nk = g.Kp / OZ_BK;
-    issue(stage0, 0);
+    // this workgroup's K steps: [k0, k0 + nk)
+    const int64_t k0 = (int64_t)(blockIdx.x % g.sk) * g.kslice;
+    const int64_t nk = min(g.Kp / OZ_BK - k0, g.kslice);
+    issue(stage0, k0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int64_t kt = 0; kt < nk; kt += 2) {
-        if (kt + 1 < nk) issue(stage1, kt + 1);
+        if (kt + 1 < nk) issue(stage1, k0 + kt + 1);
         step(stage0);
         after(kt + 1 < nk);
         if (kt + 1 >= nk) break;
-        if (kt + 2 < nk) issue(stage0, kt + 2);
+        if (kt + 2 < nk) issue(stage0, k0 + kt + 2);
         step(stage1);
         after(kt + 2 < nk);
     }
-    oz_flush<DTCF_PREC_I8, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+    oz_flush<DTCF_PREC_I8, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane, wsp, first);
 }
 
 // The same computation with a deeper load pipeline, for NS <= 4: LDS holds
@@ -970,7 +988,11 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
     // s_waitcnt with vmcnt = PH and no expcnt / lgkmcnt wait (GFX9 encoding:
     // vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[15:14])
     constexpr int WAIT_PH = (PH & 15) | (7 << 4) | (15 << 8) | ((PH >> 4) << 14);
-    const int64_t nk = g.Kp / OZ_BK;
+    // this workgroup's K steps: [k0, k0 + nk) (k below counts from k0)
+    const int64_t k0 = (int64_t)(blockIdx.x % g.sk) * g.kslice;
+    const int64_t nk = min(g.Kp / OZ_BK - k0, g.kslice);
+    double *const wsp = g.ws ? g.ws + (int64_t)blockIdx.x * OZ_BM * OZ_BN : nullptr;
+    bool first = true;
     int64_t since = 0;
     // Workgroup barrier without __syncthreads' release fence, which would
     // wait for every outstanding load (vmcnt 0) and so for A_{k+2} too.  The
@@ -984,7 +1006,8 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         else __builtin_amdgcn_s_waitcnt(0);
         barrier();
         if (++since == g.kchunk_steps && k + 1 < nk) {
-            oz_flush<P, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+            oz_flush<P, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane, wsp, first);
+            first = false;
 #pragma unroll
             for (int l = 0; l < L - 1; l++)
 #pragma unroll
@@ -995,10 +1018,10 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         }
     };
 
-    issue(h0, g.A, pa, m0, 0);
-    issue(h1, g.B, pb, n0, 0);
+    issue(h0, g.A, pa, m0, k0);
+    issue(h1, g.B, pb, n0, k0);
     if (nk > 1) {
-        issue(h2, g.A, pa, m0, 1);
+        issue(h2, g.A, pa, m0, k0 + 1);
         __builtin_amdgcn_s_waitcnt(WAIT_PH);
     } else {
         __builtin_amdgcn_s_waitcnt(0);
@@ -1008,8 +1031,8 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
     {                                                     \
         const int64_t k = kt + (J);                       \
         if (k >= nk) break;                               \
-        if (k + 1 < nk) issue(NB, g.B, pb, n0, k + 1);    \
-        if (k + 2 < nk) issue(NA, g.A, pa, m0, k + 2);    \
+        if (k + 1 < nk) issue(NB, g.B, pb, n0, k0 + k + 1); \
+        if (k + 2 < nk) issue(NA, g.A, pa, m0, k0 + k + 2); \
         oz_step<P, NS, L>(acc, SA, SB, wm, wn, lane);     \
         after(k);                                         \
     }
@@ -1021,7 +1044,32 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
         OZ_RING_STEP(h3, h4, h1, h2, 4)
     }
 #undef OZ_RING_STEP
-    oz_flush<P, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane);
+    oz_flush<P, NS, L, CVIEW>(g, acc, m0, n0, wm, wn, lane, wsp, first);
+}
+
+// Adds the K slices of every tile (workspace slot tile * sk + slice, alpha
+// already applied) to C in slice order: deterministic.
+template <bool CVIEW>
+__global__ __launch_bounds__(256) void oz_split_reduce(OzArgs g)
+{
+    const uint32_t q = blockIdx.y;  // tile, in the order oz_tile gives a split launch
+    const uint32_t width = 8 * g.tiles_n;
+    const uint32_t first = (q / width) * 8;
+    const uint32_t gsz = min(g.tiles_m - first, 8u);
+    const uint32_t tm = first + (q % width) % gsz, tn = (q % width) / gsz;
+    const int64_t m0 = (int64_t)tm * OZ_BM, n0 = (int64_t)tn * OZ_BN;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < OZ_BM * OZ_BN; e += gridDim.x * 256) {
+        const int64_t row = m0 + e / OZ_BN, col = n0 + e % OZ_BN;
+        if (row >= g.M || col >= g.N) continue;
+        const double *w = g.ws + (int64_t)q * g.sk * OZ_BM * OZ_BN + e;
+        double sum = 0.0;
+        for (uint32_t sl = 0; sl < g.sk; sl++) sum += w[(int64_t)sl * OZ_BM * OZ_BN];
+        int64_t addr;
+        if constexpr (CVIEW)
+            addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + g.cv.cofs[col];
+        else addr = row * g.ldc + col;
+        g.C[addr] += sum;
+    }
 }
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
@@ -1033,7 +1081,7 @@ int g_fused_i8 = 1;
 template <int P, int NS, int L>
 int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
 {
-    const unsigned grid = g.tiles_m * g.tiles_n;
+    const unsigned grid = g.tiles_m * g.tiles_n * g.sk;
     if constexpr (NS <= 4) {
         if (cview) hipLaunchKernelGGL((ozaki_ring_kernel<P, NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
         else hipLaunchKernelGGL((ozaki_ring_kernel<P, NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
@@ -1043,6 +1091,12 @@ int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
         else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
     }
     DTCF_HIP(hipGetLastError());
+    if (g.sk > 1) {
+        const dim3 rg(OZ_BM * OZ_BN / 1024, g.tiles_m * g.tiles_n);
+        if (cview) hipLaunchKernelGGL(oz_split_reduce<true>, rg, dim3(256), 0, s, g);
+        else hipLaunchKernelGGL(oz_split_reduce<false>, rg, dim3(256), 0, s, g);
+        DTCF_HIP(hipGetLastError());
+    }
     return DTCF_OK;
 }
 
@@ -1295,6 +1349,30 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
         }
         const int64_t kmax = ((int64_t(1) << 31) - 1) / worst;
         o.kchunk_steps = prec == DTCF_PREC_I8 ? std::max<int64_t>(1, kmax / OZ_BK) : INT64_MAX;
+        // K split when the tiles fill under half the CUs (one workgroup per
+        // CU): sk slices of >= 16 K steps, sk * tiles <= CUs, so the slices
+        // (alpha * partial C tiles) fit the stream's GEMM split workspace.
+        const uint32_t tiles = o.tiles_m * o.tiles_n;
+        const int64_t nkall = o.Kp / OZ_BK;
+        o.sk = 1;
+        o.kslice = nkall;
+        o.ws = nullptr;
+        int cus = 0;
+        int sst = gemm_device_cus(&cus);
+        if (sst != DTCF_OK) return sst;
+        if (tiles < (uint32_t)cus / 2 && nkall >= 32) {
+            int64_t sk = std::min<int64_t>(cus / tiles, nkall / 16);
+            if (sk >= 2) {
+                const int64_t ks = (nkall + sk - 1) / sk;
+                sk = (nkall + ks - 1) / ks;
+                double *ws = nullptr;
+                sst = gemm_workspace(s, (size_t)tiles * sk * OZ_BM * OZ_BN * sizeof(double), &ws);
+                if (sst != DTCF_OK) return sst;
+                o.sk = (uint32_t)sk;
+                o.kslice = ks;
+                o.ws = ws;
+            }
+        }
         bool done = false;
         int st = oz_dispatch(prec, ns, lev, o, cv != nullptr, s, &done);
         if (st != DTCF_OK || done) return st;

@@ -106,10 +106,11 @@ def test_device_generators_match_host(dev):
                                          (130, 70, 70000, False), (48, 96, 70000, True)])
 def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
     """The fused kernels sum each level exactly in INT32 and add the levels to
-    C at most every 32,640 (five digits to level 6) or 43,456 (otherwise) K --
-    K = 70,000 takes one or two flushes inside the loop -- while the
-    per-product path adds every product to C.  Same digits, so the results
-    agree to FP64 rounding."""
+    C at most every 32,640 (five digits to level 6) or 43,456 (otherwise) K;
+    these grids have a few C tiles, so K is also cut into slices whose
+    partial tiles a second kernel adds in order -- while the per-product path
+    adds every product to C.  Same digits, so the results agree to FP64
+    rounding."""
     rng = np.random.default_rng(m * n + k + ns)
     A = rng.standard_normal((m, k))
     B = rng.standard_normal((n, k))
@@ -184,30 +185,25 @@ def test_fused_int8_flush_interval_at_largest_digits(dev, ns, lev):
     positive and each level sum grows by 98-99% of the per-K bound the flush
     interval is sized for.  A bound that left out a product or undercounted a
     digit (so the interval too long) overflows INT32 here; the exact result is
-    alpha K x^2 up to the dropped levels."""
-    m, n, k = 130, 70, 70000
+    alpha K x^2 up to the dropped levels.  132 C tiles: enough for the fused
+    kernel to run without a K split, so every workgroup sums all K = 50,000
+    and flushes inside its loop."""
+    m, n, k = 1536, 1408, 50000
     X = -(63 * 256 ** (ns - 1) + 128 * (256 ** (ns - 1) - 1) // 255)
     x = X / 2.0 ** (8 * ns - 2)
     Ad = torch.full((m, k), x, dtype=torch.float64, device=dev)
     Bd = torch.full((n, k), x, dtype=torch.float64, device=dev)
     sa = mixed.Split(mixed.I8, Ad, mixed.dense_view(m, k), nslice=ns)
     sb = mixed.Split(mixed.I8, Bd, mixed.dense_view(n, k), nslice=ns)
-    out = []
-    try:
-        for fused in (True, False):
-            mixed.set_fused(fused)
-            Cd = torch.zeros(m * n, dtype=torch.float64, device=dev)
-            mixed.gemm_split(sa, sb, Cd, alpha=0.5, ldc=n, max_level=lev)
-            out.append(Cd.cpu().numpy().reshape(m, n))
-    finally:
-        mixed.set_fused(True)
+    Cd = torch.zeros(m * n, dtype=torch.float64, device=dev)
+    mixed.gemm_split(sa, sb, Cd, alpha=0.5, ldc=n, max_level=lev)
+    out = Cd.cpu().numpy()
     # levels above lev dropped: the largest is 2^(-8 (lev + 1)) * 2^4 * (products) * K
     ref = 0.5 * k * x * x
     drop = 0.5 * k * 16 * ns * 128 * 128 * 2.0 ** (-8 * (lev + 1))
-    print(f"ns={ns} L={lev}: fused {out[0][0, 0]!r} per-product {out[1][0, 0]!r} exact {ref!r}")
-    assert np.all(out[0] == out[0][0, 0]) and np.all(out[1] == out[1][0, 0])
-    assert abs(out[0][0, 0] - out[1][0, 0]) <= 1e-14 * ref
-    assert abs(out[0][0, 0] - ref) <= drop + 1e-14 * ref
+    print(f"ns={ns} L={lev}: fused {out[0]!r} exact {ref!r}")
+    assert np.all(out == out[0])
+    assert abs(out[0] - ref) <= drop + 1e-14 * ref
 
 
 @pytest.mark.parametrize("prec", [mixed.F32, mixed.BF16X3, mixed.F16X3])
