@@ -211,6 +211,8 @@ struct PStmt {
     int64_t ldc = 0;
     int64_t scratch_off = -1, scratch_bytes = 0;
     int64_t sa_bytes = 0, sb_bytes = 0, ea_off = 0, eb_off = 0;
+    // the contraction runs in the program's mixed precision (else FP64)
+    bool mixed = false;
     // FP64: operands in GEMM position 0 (A) / 1 (B) packed into scratch first
     bool pack[2] = {false, false};
     dtcf_strided pdesc[2];
@@ -228,6 +230,8 @@ struct PStmt {
 struct dtcf_program_s {
     dtcf_server srv = nullptr;
     int prec = 0, nslice = 0, max_level = 0;
+    // mixed precision only for contractions with m n / (m + n) >= mixed_min
+    double mixed_min = 256.0;
     std::vector<PArr> arrs;
     std::map<std::string, int> byname;
     std::vector<PStmt> st;
@@ -398,12 +402,18 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
     s.m = (int64_t)Ar.size();
     s.n = (int64_t)Br.size();
     s.k = (int64_t)Ak.size();
+    // Splitting both operands costs about (m + n) k (16 + S) bytes of HBM
+    // traffic; the slices save about two thirds of the FP64 GEMM's 2 m n k
+    // flops.  With 66 TF/s and 5 TB/s that pays off from m n / (m + n) ~ 200:
+    // thinner contractions (the T1 terms, n = o) stay FP64.
+    s.mixed = p->prec != 0 &&
+              (double)s.m * (double)s.n >= p->mixed_min * (double)(s.m + s.n);
     // FP64: an operand with neither direction contiguous is permuted into a
     // dense [rows][K] copy first (one pass at HBM speed) unless it is small.
     // Mixed precision re-lays operands out in its split pass anyway.
     int64_t scr = 0;
     std::vector<int64_t> dense_r[2], dense_k[2];
-    for (int q = 0; q < 2 && p->prec == 0; q++) {
+    for (int q = 0; q < 2 && !s.mixed; q++) {
         const auto &R = q == 0 ? Ar : Br;
         const auto &Kk = q == 0 ? Ak : Bk;
         const int64_t rows = (int64_t)R.size(), K = (int64_t)Kk.size();
@@ -451,7 +461,7 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
     }
     s.scratch_bytes = scr;
     s.flops = 2.0 * (double)s.m * (double)s.n * (double)s.k;
-    if (p->prec != 0) {
+    if (s.mixed) {
         s.sa_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.m, s.k));
         s.sb_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.n, s.k));
         DTCF_REQUIRE(s.sa_bytes > 0 && s.sb_bytes > 0, "program: bad mixed-precision mode");
@@ -495,7 +505,7 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
     }
     const double *A = arr_ptr(p, p->arrs[s.a], r), *B = arr_ptr(p, p->arrs[s.b], r);
     if (s.swap) std::swap(A, B);
-    if (p->prec == 0) {
+    if (!s.mixed) {
         // FP64: the GEMM writes C = A * B directly (no clearing pass)
         const double *op[2] = {A, B};
         for (int q = 0; q < 2; q++) {
@@ -1034,6 +1044,14 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
         p->flops += s.flops;
     }
     p->final = true;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_program_set_mixed_min(dtcf_program p, double h)
+{
+    DTCF_REQUIRE(p && !p->final && p->st.empty(), "program: set_mixed_min before any statement");
+    DTCF_REQUIRE(h >= 0.0, "program: negative mixed_min");
+    p->mixed_min = h;
     return DTCF_OK;
 }
 

@@ -441,10 +441,15 @@ int dtcf_server_usage(dtcf_server s, int64_t *budget, int64_t *resident, int64_t
 /* A program is the statement list of one term file over one system's
  * extents (a CC residual evaluation).  Arrays are dense row-major FP64.
  * Index lists are comma-separated index names, e.g. "p1a,h2a".
- * prec: 0 = FP64 MFMA, else a DTCF_PREC_* mode for every contraction
- * (nslice, max_level as in dtcf_gemm_split_f64); sums stay FP64. */
+ * prec: 0 = FP64 MFMA, else a DTCF_PREC_* mode (nslice, max_level as in
+ * dtcf_gemm_split_f64) for every contraction C[m x n] with m n / (m + n) >=
+ * mixed_min (default 256: splitting both operands costs less HBM time than
+ * the slices save; thinner contractions run FP64); sums stay FP64. */
 enum { DTCF_ARR_INPUT = 0, DTCF_ARR_TEMP = 1, DTCF_ARR_OUTPUT = 2 };
 int dtcf_program_create(dtcf_server s, int prec, int nslice, int max_level, dtcf_program *out);
+/* Before the first statement: the m n / (m + n) from which a contraction
+ * takes the program's mixed precision (0: every contraction). */
+int dtcf_program_set_mixed_min(dtcf_program p, double h);
 int dtcf_program_array(dtcf_program p, const char *name, int ndim, const int64_t *dims, int role);
 /* Inputs are resident tensors the caller owns (e.g. from dtcf_server_alloc). */
 int dtcf_program_bind(dtcf_program p, const char *name, const double *dev_ptr);

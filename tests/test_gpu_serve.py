@@ -88,7 +88,7 @@ def test_mixed_program_against_fp64(dev, lists, prec, ns, tol):
     with S.Server(1 << 30, nslots=4) as srv:
         sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
         p64 = S.Program(srv, tl, sysm)
-        pmx = S.Program(srv, tl, sysm, prec=prec, nslice=ns)
+        pmx = S.Program(srv, tl, sysm, prec=prec, nslice=ns, mixed_min=0)  # every contraction
         r64, rmx = p64.run(), pmx.run()
         for name in p64.outputs:
             err = np.abs(rmx[name] - r64[name]).max() / np.abs(r64[name]).max()
@@ -96,6 +96,26 @@ def test_mixed_program_against_fp64(dev, lists, prec, ns, tol):
             assert err <= tol, (name, err)
         p64.destroy()
         pmx.destroy()
+
+
+def test_mixed_min_policy(dev, lists):
+    """A mixed program takes FP64 for contractions with m n / (m + n) below
+    mixed_min: at the default (256) every contraction of a small system is
+    one, and its results equal the FP64 program's bitwise; with mixed_min 0
+    they take INT8 digit slices (and differ, within the INT8x5 bound)."""
+    tl = lists["t2"]
+    sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 10}
+    with S.Server(1 << 30, nslots=2) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+        p64 = S.Program(srv, tl, sysm)
+        pdef = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5)
+        pall = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0)
+        r64, rdef, rall = p64.run(), pdef.run(), pall.run()
+        for name in p64.outputs:
+            assert np.array_equal(rdef[name], r64[name]), name
+        assert any(not np.array_equal(rall[n], r64[n]) for n in p64.outputs)
+        for p in (p64, pdef, pall):
+            p.destroy()
 
 
 def _outputs_host(outs):
@@ -113,7 +133,7 @@ def test_alone_vs_mix_bitwise(dev, lists):
         progs = {
             "A2": S.Program(srv, t2, A), "A1": S.Program(srv, t1, A),
             "B2": S.Program(srv, t2, B), "B1": S.Program(srv, t1, B),
-            "A2i8": S.Program(srv, t2, A, prec=mixed.I8, nslice=5),
+            "A2i8": S.Program(srv, t2, A, prec=mixed.I8, nslice=5, mixed_min=0),
         }
         torch.cuda.synchronize()
         alone = {}
