@@ -244,14 +244,19 @@ def leg_c1(dev, log, reps=20):
 def leg_serving(log, requests=36, rate=2.5):
     """Served CCSD iteration stream: UHF T1 + T2 programs over o/v in
     {(32,128), (64,256), (96,384)}, Poisson arrivals, 200 GB HBM budget."""
-    from dtcf_amd import servebench
-    srv, progs = servebench.build(200.0, 8, log=lambda s_: log("[serving] " + s_))
+    from dtcf_amd import mixed, servebench
+    srv, allp = servebench.build(200.0, 8, precs=((0, 0), (mixed.I8, 5)),
+                                 log=lambda s_: log("[serving] " + s_))
+    progs = [p for p in allp if p.label.endswith("fp64")]
     try:
-        alone = servebench.service_times(srv, progs)
+        alone = servebench.service_times(srv, allp)
         res = servebench.run_stream(srv, progs, requests, rate)
         sat = servebench.run_stream(srv, progs, 18, 0.0, seed=2)
+        # the same programs plus their INT8x5 forms (contractions with
+        # m n / (m + n) >= 256 on digit slices, the rest FP64), all at once
+        mix = servebench.run_stream(srv, allp, 24, 0.0, seed=3)
     finally:
-        for p in progs:
+        for p in allp:
             p.destroy()
         srv.close()
     out = {"workload": "UHF T1+T2 residual programs (72 + 306 statements), systems o/v in "
@@ -261,6 +266,9 @@ def leg_serving(log, requests=36, rate=2.5):
                                           "request_latency_ms")},
            "saturated": {k: sat[k] for k in ("requests", "statements", "seconds", "tflops",
                                             "requests_per_s", "task_latency_ms", "request_latency_ms")},
+           "saturated_with_int8x5": {k: mix[k] for k in ("requests", "statements", "seconds", "tflops",
+                                                        "requests_per_s", "task_latency_ms",
+                                                        "request_latency_ms")},
            "alone_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in alone.items()},
            "alone_tflops": {k: round(v["tflops"], 2) for k, v in alone.items()},
            "peak_reserved_gb": round(res["usage"]["peak_reserved"] / 1e9, 2),
