@@ -255,6 +255,14 @@ def leg_serving(log, requests=36, rate=2.5):
         # the same programs plus their INT8x5 forms (contractions with
         # m n / (m + n) >= 256 on digit slices, the rest FP64), all at once
         mix = servebench.run_stream(srv, allp, 24, 0.0, seed=3)
+        # error of the INT8x5 residuals against the FP64 program's, o=64 v=256
+        byl = {p.label: p for p in allp}
+        err = {}
+        for which in ("t1", "t2"):
+            r64 = byl[f"{which}/o64v256/fp64"].run()
+            rmx = byl[f"{which}/o64v256/int8x5"].run()
+            err[which] = max(float(np.abs(rmx[n] - r64[n]).max() / max(np.abs(r64[n]).max(), 1e-300))
+                             for n in r64)
     finally:
         for p in allp:
             p.destroy()
@@ -269,6 +277,7 @@ def leg_serving(log, requests=36, rate=2.5):
            "saturated_with_int8x5": {k: mix[k] for k in ("requests", "statements", "seconds", "tflops",
                                                         "requests_per_s", "task_latency_ms",
                                                         "request_latency_ms")},
+           "int8x5_max_rel_err_o64v256": {k: sig(v) for k, v in err.items()},
            "alone_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in alone.items()},
            "alone_tflops": {k: round(v["tflops"], 2) for k, v in alone.items()},
            "peak_reserved_gb": round(res["usage"]["peak_reserved"] / 1e9, 2),
