@@ -210,7 +210,12 @@ struct PStmt {
     dtcf_view av = nullptr, bv = nullptr, cv = nullptr;
     int64_t ldc = 0;
     int64_t scratch_off = -1, scratch_bytes = 0;
-    int64_t sa_bytes = 0, sb_bytes = 0, ea_off = 0, eb_off = 0;
+    int64_t sa_bytes = 0, sb_bytes = 0, ea_off = 0, eb_off = 0, a_off = 0, b_off = 0;
+    // mixed: GEMM operand q taken from slices made once at finalize (an input
+    // array, resident slices on): rs[q] slices, re[q] row exponents
+    bool res[2] = {false, false};
+    char *rs[2] = {nullptr, nullptr};
+    int32_t *re[2] = {nullptr, nullptr};
     // the contraction runs in the program's mixed precision (else FP64)
     bool mixed = false;
     // FP64: operands in GEMM position 0 (A) / 1 (B) packed into scratch first
@@ -232,6 +237,9 @@ struct dtcf_program_s {
     int prec = 0, nslice = 0, max_level = 0;
     // mixed precision only for contractions with m n / (m + n) >= mixed_min
     double mixed_min = 256.0;
+    // slices of input operands made once at finalize and kept (server arena)
+    bool resident_slices = false;
+    std::vector<void *> res_allocs;
     std::vector<PArr> arrs;
     std::map<std::string, int> byname;
     std::vector<PStmt> st;
@@ -465,9 +473,16 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
         s.sa_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.m, s.k));
         s.sb_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.n, s.k));
         DTCF_REQUIRE(s.sa_bytes > 0 && s.sb_bytes > 0, "program: bad mixed-precision mode");
-        s.ea_off = s.sa_bytes + s.sb_bytes;
-        s.eb_off = s.ea_off + granule((s.m + 127) / 128 * 128 * 4);
-        s.scratch_bytes = s.eb_off + granule((s.n + 127) / 128 * 128 * 4);
+        // GEMM operand 0 is array b when swapped, a otherwise
+        const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
+        for (int q = 0; q < 2; q++)
+            s.res[q] = p->resident_slices && p->arrs[arr[q]].role == DTCF_ARR_INPUT;
+        int64_t off = 0;
+        if (!s.res[0]) { s.a_off = off; off += s.sa_bytes; }
+        if (!s.res[1]) { s.b_off = off; off += s.sb_bytes; }
+        if (!s.res[0]) { s.ea_off = off; off += granule((s.m + 127) / 128 * 128 * 4); }
+        if (!s.res[1]) { s.eb_off = off; off += granule((s.n + 127) / 128 * 128 * 4); }
+        s.scratch_bytes = off;
     }
     return DTCF_OK;
 }
@@ -523,13 +538,15 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(C.elems)), dim3(256), 0, stream, c, C.elems);
     DTCF_HIP(hipGetLastError());
     char *scr = r->base + s.scratch_off;
-    int32_t *ea = reinterpret_cast<int32_t *>(scr + s.ea_off);
-    int32_t *eb = reinterpret_cast<int32_t *>(scr + s.eb_off);
-    int st = dtcf_split_f64(p->prec, p->nslice, A, s.av, scr, ea, stream);
-    if (st == DTCF_OK) st = dtcf_split_f64(p->prec, p->nslice, B, s.bv, scr + s.sa_bytes, eb, stream);
+    char *sa = s.res[0] ? s.rs[0] : scr + s.a_off, *sb = s.res[1] ? s.rs[1] : scr + s.b_off;
+    int32_t *ea = s.res[0] ? s.re[0] : reinterpret_cast<int32_t *>(scr + s.ea_off);
+    int32_t *eb = s.res[1] ? s.re[1] : reinterpret_cast<int32_t *>(scr + s.eb_off);
+    int st = DTCF_OK;
+    if (!s.res[0]) st = dtcf_split_f64(p->prec, p->nslice, A, s.av, sa, ea, stream);
+    if (st == DTCF_OK && !s.res[1]) st = dtcf_split_f64(p->prec, p->nslice, B, s.bv, sb, eb, stream);
     if (st == DTCF_OK)
-        st = dtcf_gemm_split_f64(p->prec, p->nslice, s.m, s.n, s.k, s.alpha, scr, ea,
-                                 scr + s.sa_bytes, eb, p->max_level, c, s.cv, s.ldc, stream);
+        st = dtcf_gemm_split_f64(p->prec, p->nslice, s.m, s.n, s.k, s.alpha, sa, ea, sb, eb,
+                                 p->max_level, c, s.cv, s.ldc, stream);
     return st;
 }
 
@@ -1043,7 +1060,41 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
         }
         p->flops += s.flops;
     }
+    // Resident slices: each mixed contraction's input operands split once,
+    // from the bound inputs as they are now, into the server's arena.
+    for (auto &s : p->st) {
+        if (s.kind != 0 || !s.mixed) continue;
+        const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
+        for (int q = 0; q < 2; q++) {
+            if (!s.res[q]) continue;
+            const int64_t rows = q == 0 ? s.m : s.n;
+            const int64_t sbytes = q == 0 ? s.sa_bytes : s.sb_bytes;
+            void *ptr = nullptr;
+            int st = dtcf_server_alloc(p->srv, sbytes + granule((rows + 127) / 128 * 128 * 4), &ptr);
+            if (st != DTCF_OK) {
+                p->broken = true;  // deps are already linked: no second finalize
+                return st;
+            }
+            p->res_allocs.push_back(ptr);
+            s.rs[q] = static_cast<char *>(ptr);
+            s.re[q] = reinterpret_cast<int32_t *>(s.rs[q] + sbytes);
+            st = dtcf_split_f64(p->prec, p->nslice, p->arrs[arr[q]].bound, q == 0 ? s.av : s.bv, s.rs[q],
+                                s.re[q], nullptr);
+            if (st != DTCF_OK) {
+                p->broken = true;
+                return st;
+            }
+        }
+    }
+    if (!p->res_allocs.empty()) DTCF_HIP(hipStreamSynchronize(nullptr));
     p->final = true;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_program_set_resident_slices(dtcf_program p, int on)
+{
+    DTCF_REQUIRE(p && !p->final && p->st.empty(), "program: set_resident_slices before any statement");
+    p->resident_slices = on != 0;
     return DTCF_OK;
 }
 
@@ -1096,6 +1147,7 @@ extern "C" int dtcf_program_destroy(dtcf_program p)
         if (s.cv) dtcf_view_destroy(s.cv);
         if (s.dstrides) (void)hipFree(s.dstrides);
     }
+    for (void *ptr : p->res_allocs) (void)dtcf_server_free(p->srv, ptr);
     delete p;
     return DTCF_OK;
 }

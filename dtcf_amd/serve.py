@@ -49,6 +49,7 @@ def _declare():
         "dtcf_program_finalize": [vp],
         "dtcf_program_set_layout": [vp, C.c_int],
         "dtcf_program_set_mixed_min": [vp, C.c_double],
+        "dtcf_program_set_resident_slices": [vp, C.c_int],
         "dtcf_program_get_layout": [vp, C.POINTER(C.c_int)],
         "dtcf_program_info": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(dbl), C.POINTER(i64)],
         "dtcf_program_destroy": [vp],
@@ -208,7 +209,8 @@ class Program:
     placement of temporaries (LAYOUT_*; ``self.layout`` is the one used)."""
 
     def __init__(self, server: Server, tl: termlist.TermList, system: System, prec=0, nslice=5,
-                 max_level=0, expose=False, layout=LAYOUT_AUTO, mixed_min=None):
+                 max_level=0, expose=False, layout=LAYOUT_AUTO, mixed_min=None,
+                 resident_slices=False):
         self.server, self.tl, self.system = server, tl, system
         L = lib()
         h = C.c_void_p()
@@ -216,6 +218,8 @@ class Program:
         self.h = h
         if mixed_min is not None:
             check(L.dtcf_program_set_mixed_min(h, float(mixed_min)))
+        if resident_slices:
+            check(L.dtcf_program_set_resident_slices(h, 1))
         self.prec = prec
         used = []
         for s in tl.statements:

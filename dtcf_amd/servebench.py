@@ -23,7 +23,8 @@ from ._lib import check, lib
 SYSTEMS = ((32, 128), (64, 256), (96, 384))
 
 
-def build(budget_gb=200.0, nslots=8, systems=SYSTEMS, precs=((0, 0),), log=print):
+def build(budget_gb=200.0, nslots=8, systems=SYSTEMS, precs=((0, 0),), log=print,
+          resident_slices=False):
     t1, t2 = termlist.load("t1"), termlist.load("t2")
     srv = serve.Server(int(budget_gb * (1 << 30)), nslots)
     progs = []
@@ -32,11 +33,12 @@ def build(budget_gb=200.0, nslots=8, systems=SYSTEMS, precs=((0, 0),), log=print
         sysm = serve.System(srv, o, v, [t1, t2], seed=o * 1000 + v)
         for prec, ns in precs:
             for which, tl in (("t1", t1), ("t2", t2)):
-                p = serve.Program(srv, tl, sysm, prec=prec, nslice=ns)
+                p = serve.Program(srv, tl, sysm, prec=prec, nslice=ns,
+                                  resident_slices=resident_slices and prec != 0)
                 p.label = f"{which}/o{o}v{v}/" + ("fp64" if prec == 0 else mixed.NAMES[prec] + f"x{ns}")
                 progs.append(p)
         log(f"system o={o} v={v}: inputs {sysm.bytes() / 1e9:.2f} GB, set up in "
-            f"{time.perf_counter() - t:.2f} s")
+            f"{time.perf_counter() - t:.2f} s; server resident {srv.usage()['resident'] / 1e9:.2f} GB")
     return srv, progs
 
 
@@ -112,12 +114,15 @@ def main():
     ap.add_argument("--requests", type=int, default=36)
     ap.add_argument("--rate", type=float, default=2.5)
     ap.add_argument("--mixed", action="store_true", help="add INT8x5 programs to the mix")
+    ap.add_argument("--resident-slices", action="store_true",
+                    help="mixed programs split their input operands once, at finalize")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     check(lib().dtcf_init(0))
     precs = ((0, 0), (mixed.I8, 5)) if args.mixed else ((0, 0),)
     log = lambda s: print(s, flush=True)
-    srv, progs = build(args.budget_gb, args.slots, precs=precs, log=log)
+    srv, progs = build(args.budget_gb, args.slots, precs=precs, log=log,
+                       resident_slices=args.resident_slices)
     alone = service_times(srv, progs)
     for k, v in alone.items():
         log(f"alone {k}: {v['seconds'] * 1e3:.1f} ms, {v['tflops']:.1f} TF/s, block {v['block_gb']:.2f} GB")

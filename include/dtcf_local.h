@@ -450,6 +450,12 @@ int dtcf_program_create(dtcf_server s, int prec, int nslice, int max_level, dtcf
 /* Before the first statement: the m n / (m + n) from which a contraction
  * takes the program's mixed precision (0: every contraction). */
 int dtcf_program_set_mixed_min(dtcf_program p, double h);
+/* Before the first statement, mixed programs: 1 makes finalize split every
+ * input operand of a mixed contraction once, from the bound input's contents
+ * at that time, into the server's arena (counted as resident until the
+ * program is destroyed); requests then split only the other operands.  The
+ * inputs must be written before finalize and not change afterwards. */
+int dtcf_program_set_resident_slices(dtcf_program p, int on);
 int dtcf_program_array(dtcf_program p, const char *name, int ndim, const int64_t *dims, int role);
 /* Inputs are resident tensors the caller owns (e.g. from dtcf_server_alloc). */
 int dtcf_program_bind(dtcf_program p, const char *name, const double *dev_ptr);

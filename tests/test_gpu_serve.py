@@ -118,6 +118,28 @@ def test_mixed_min_policy(dev, lists):
             p.destroy()
 
 
+def test_resident_slices_bitwise(dev, lists):
+    """Input operands split once at finalize give bitwise the results of
+    splitting them in every request (same kernels, same digits), twice in a
+    row; the slices count as resident and are released with the program."""
+    tl = lists["t2"]
+    sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 10}
+    with S.Server(1 << 30, nslots=2) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+        base = srv.usage()["resident"]
+        p0 = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0)
+        p1 = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0, resident_slices=True)
+        assert srv.usage()["resident"] > base
+        r0 = p0.run()
+        for _ in range(2):
+            r1 = p1.run()
+            for name in p0.outputs:
+                assert np.array_equal(r0[name], r1[name]), name
+        p0.destroy()
+        p1.destroy()
+        assert srv.usage()["resident"] == base
+
+
 def _outputs_host(outs):
     return [o.cpu().numpy().copy() for o in outs]
 
