@@ -453,8 +453,9 @@ int dtcf_program_set_mixed_min(dtcf_program p, double h);
 /* Before the first statement, mixed programs: 1 makes finalize split every
  * input operand of a mixed contraction once, from the bound input's contents
  * at that time, into the server's arena (counted as resident until the
- * program is destroyed); requests then split only the other operands.  The
- * inputs must be written before finalize and not change afterwards. */
+ * program is destroyed, one slice set per contraction and operand view);
+ * requests then split only the other operands.  The inputs must be written
+ * before finalize and not change afterwards. */
 int dtcf_program_set_resident_slices(dtcf_program p, int on);
 int dtcf_program_array(dtcf_program p, const char *name, int ndim, const int64_t *dims, int role);
 /* Inputs are resident tensors the caller owns (e.g. from dtcf_server_alloc). */
