@@ -31,11 +31,14 @@ def build(budget_gb=200.0, nslots=8, systems=SYSTEMS, precs=((0, 0),), log=print
     for o, v in systems:
         t = time.perf_counter()
         sysm = serve.System(srv, o, v, [t1, t2], seed=o * 1000 + v)
-        for prec, ns in precs:
+        for pr in precs:
+            prec, ns = pr[0], pr[1]
+            lev = pr[2] if len(pr) > 2 else 0
             for which, tl in (("t1", t1), ("t2", t2)):
-                p = serve.Program(srv, tl, sysm, prec=prec, nslice=ns,
+                p = serve.Program(srv, tl, sysm, prec=prec, nslice=ns, max_level=lev,
                                   resident_slices=resident_slices and prec != 0)
-                p.label = f"{which}/o{o}v{v}/" + ("fp64" if prec == 0 else mixed.NAMES[prec] + f"x{ns}")
+                p.label = f"{which}/o{o}v{v}/" + ("fp64" if prec == 0 else
+                                                   mixed.NAMES[prec] + f"x{ns}" + (f"L{lev}" if lev else ""))
                 progs.append(p)
         log(f"system o={o} v={v}: inputs {sysm.bytes() / 1e9:.2f} GB, set up in "
             f"{time.perf_counter() - t:.2f} s; server resident {srv.usage()['resident'] / 1e9:.2f} GB")
@@ -114,12 +117,15 @@ def main():
     ap.add_argument("--requests", type=int, default=36)
     ap.add_argument("--rate", type=float, default=2.5)
     ap.add_argument("--mixed", action="store_true", help="add INT8x5 programs to the mix")
+    ap.add_argument("--int8x4l6", action="store_true",
+                    help="with --mixed: four digits, levels up to 6, instead of INT8x5")
     ap.add_argument("--resident-slices", action="store_true",
                     help="mixed programs split their input operands once, at finalize")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     check(lib().dtcf_init(0))
-    precs = ((0, 0), (mixed.I8, 5)) if args.mixed else ((0, 0),)
+    precs = (((0, 0), (mixed.I8, 4, 6) if args.int8x4l6 else (mixed.I8, 5)) if args.mixed
+             else ((0, 0),))
     log = lambda s: print(s, flush=True)
     srv, progs = build(args.budget_gb, args.slots, precs=precs, log=log,
                        resident_slices=args.resident_slices)

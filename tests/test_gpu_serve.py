@@ -79,16 +79,17 @@ def test_concurrent_program_matches_oracle(dev, lists, which, layout):
         prog.destroy()
 
 
-@pytest.mark.parametrize("prec,ns,tol", [(mixed.I8, 5, 5e-8), (mixed.F32, 1, 5e-5),
-                                         (mixed.BF16X3, 3, 5e-5)])
-def test_mixed_program_against_fp64(dev, lists, prec, ns, tol):
+@pytest.mark.parametrize("prec,ns,lev,tol", [(mixed.I8, 5, 0, 5e-8), (mixed.I8, 4, 6, 5e-7),
+                                             (mixed.F32, 1, 0, 5e-5), (mixed.BF16X3, 3, 0, 5e-5)])
+def test_mixed_program_against_fp64(dev, lists, prec, ns, lev, tol):
     tl = lists["t2"]
     sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 10}
     print(f"serve mixed prec={prec} ns={ns}", flush=True)
     with S.Server(1 << 30, nslots=4) as srv:
         sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
         p64 = S.Program(srv, tl, sysm)
-        pmx = S.Program(srv, tl, sysm, prec=prec, nslice=ns, mixed_min=0)  # every contraction
+        pmx = S.Program(srv, tl, sysm, prec=prec, nslice=ns, max_level=lev,
+                        mixed_min=0)  # every contraction
         r64, rmx = p64.run(), pmx.run()
         for name in p64.outputs:
             err = np.abs(rmx[name] - r64[name]).max() / np.abs(r64[name]).max()
