@@ -392,7 +392,11 @@ __global__ __launch_bounds__(512) void row_exp_kernel(const double *__restrict__
     }
 }
 
-template <int MODE>
+// RUN (rows contiguous in runs of 16, K strided): sixteen consecutive
+// threads take the 16 rows of one chunk, so each of their loads reads one
+// whole cache line of 16 rows; otherwise consecutive threads take
+// consecutive chunks of one row (K contiguous).
+template <int MODE, bool RUN>
 __global__ __launch_bounds__(256) void split_wide_kernel(const double *__restrict__ src, ViewArgs v,
                                                          int64_t rows, int64_t K, int64_t rows_p,
                                                          int64_t K_p, int nslice,
@@ -405,7 +409,15 @@ __global__ __launch_bounds__(256) void split_wide_kernel(const double *__restric
     const size_t plane = (size_t)rows_p * K_p * ESZ;  // bytes
     for (int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x; u < total;
          u += (int64_t)gridDim.x * 256) {
-        const int64_t row = u / cpr, c0 = (u - row * cpr) * PER;
+        int64_t row, c0;
+        if constexpr (RUN) {
+            const int64_t rest = u >> 4, rb = rest / cpr;
+            row = rb * 16 + (u & 15);
+            c0 = (rest - rb * cpr) * PER;
+        } else {
+            row = u / cpr;
+            c0 = (u - row * cpr) * PER;
+        }
         const bool real = row < rows;
         double x[PER];
 #pragma unroll
@@ -1213,28 +1225,27 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
         const int64_t chunks = rp * kp * esz / 16;
         const unsigned g2 = (unsigned)std::min<int64_t>((chunks + 255) / 256, 1 << 16);
         unsigned char *d = (unsigned char *)dst;
+        const bool run = v->rowrun16 && !v->blk;
+#define DTCF_WIDE(M)                                                                               \
+    if (run) hipLaunchKernelGGL((split_wide_kernel<M, true>), dim3(g2), dim3(256), 0, s, src, a,    \
+                                rows, K, rp, kp, ns, d, rexp);                                      \
+    else hipLaunchKernelGGL((split_wide_kernel<M, false>), dim3(g2), dim3(256), 0, s, src, a, rows, \
+                            K, rp, kp, ns, d, rexp);
         switch (prec) {
-        case DTCF_PREC_F32:
-            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_F32>, dim3(g2), dim3(256), 0, s, src, a,
-                               rows, K, rp, kp, ns, d, rexp);
-            break;
-        case DTCF_PREC_BF16X3:
-            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_BF16X3>, dim3(g2), dim3(256), 0, s, src, a,
-                               rows, K, rp, kp, ns, d, rexp);
-            break;
+        case DTCF_PREC_F32: DTCF_WIDE(DTCF_PREC_F32) break;
+        case DTCF_PREC_BF16X3: DTCF_WIDE(DTCF_PREC_BF16X3) break;
         case DTCF_PREC_F16X3:
             hipLaunchKernelGGL(row_exp_kernel<DTCF_PREC_F16X3>, dim3((unsigned)rp), dim3(512), 0, s,
                                src, a, rows, K, rexp);
-            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_F16X3>, dim3(g2), dim3(256), 0, s, src, a,
-                               rows, K, rp, kp, ns, d, rexp);
+            DTCF_WIDE(DTCF_PREC_F16X3)
             break;
         default:
             hipLaunchKernelGGL(row_exp_kernel<DTCF_PREC_I8>, dim3((unsigned)rp), dim3(512), 0, s, src,
                                a, rows, K, rexp);
-            hipLaunchKernelGGL(split_wide_kernel<DTCF_PREC_I8>, dim3(g2), dim3(256), 0, s, src, a, rows,
-                               K, rp, kp, ns, d, rexp);
+            DTCF_WIDE(DTCF_PREC_I8)
             break;
         }
+#undef DTCF_WIDE
         DTCF_HIP(hipGetLastError());
         return DTCF_OK;
     }
