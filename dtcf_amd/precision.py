@@ -10,7 +10,10 @@ T[a,b,i,j] = V_vvoo / D.  One iteration:
 where T' is T viewed as [i,k,a,c].  The two contractions run in the
 precision under study; the denominator update and the energy always run in
 FP64 on the device.  Errors are stated against the FP64 MFMA run of the same
-inputs (max |dR|, max |dR| / max |R|, |dE|).
+inputs (max |dR|, max |dR| / max |R|, |dE|), in total and per term: each
+contraction accumulates into its own buffer, and since E is linear in R the
+energy splits as E = E(V_vvoo) + E(ladder) + E(ring), each E(X) =
+1/4 sum V_oovv[i,j,a,b] X[a,b,i,j] / D_ijab.
 
 Run: python -m dtcf_amd.precision --o 64 --v 256 --out study.json
 """
@@ -87,14 +90,14 @@ class System:
         self.ring_flops = 2.0 * n * n * n
         _sync()
 
-    def iterate(self, prec, nslice, level):
-        """Returns (R, T_new, E, seconds per phase).  Phases are timed with a
-        device synchronisation on each side, so they add up to the
-        contraction time.  level: the largest digit level kept (0: nslice + 1),
-        or a pair (ladder, ring)."""
+    def iterate(self, prec, nslice, level, terms=("ladder", "ring")):
+        """Returns (R, T_new, E, seconds per phase, {term: its contribution
+        to R}).  Phases are timed with a device synchronisation on each side,
+        so they add up to the contraction time.  level: the largest digit
+        level kept (0: nslice + 1), or a pair (ladder, ring)."""
         o, v = self.o, self.v
         lev_lad, lev_ring = level if isinstance(level, tuple) else (level, level)
-        R = self.Vvvoo.clone()
+        parts = {t: torch.zeros_like(self.Vvvoo) for t in terms}
         ph = {}
 
         def timed(name, fn):
@@ -106,27 +109,40 @@ class System:
             return r
 
         if prec is None:
-            timed("gemm_ladder", lambda: ops.gemm_view(v * v, o * o, v * v, 0.5, self.Vvvvv,
-                                                       self.lad_a, self.T, self.lad_b, R, None,
-                                                       ldc=o * o))
-            timed("gemm_ring", lambda: ops.gemm_view(o * v, o * v, o * v, 1.0, self.T, self.ring_a,
-                                                     self.Wr, self.ring_b, R, self.ring_c))
+            if "ladder" in parts:
+                timed("gemm_ladder", lambda: ops.gemm_view(v * v, o * o, v * v, 0.5, self.Vvvvv,
+                                                           self.lad_a, self.T, self.lad_b,
+                                                           parts["ladder"], None, ldc=o * o))
+            if "ring" in parts:
+                timed("gemm_ring", lambda: ops.gemm_view(o * v, o * v, o * v, 1.0, self.T,
+                                                         self.ring_a, self.Wr, self.ring_b,
+                                                         parts["ring"], self.ring_c))
         else:
             # integrals (V_vvvv, W) are the same in every CC iteration: their
             # slices can stay resident; the amplitudes are split every time
-            sa = timed("split_integrals", lambda: mixed.Split(prec, self.Vvvvv, self.lad_a, nslice))
-            sb = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.lad_b, nslice))
-            timed("gemm_ladder", lambda: mixed.gemm_split(sa, sb, R, alpha=0.5, ldc=o * o,
-                                                          max_level=lev_lad))
-            del sa, sb
-            ra = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.ring_a, nslice))
-            rb = timed("split_integrals", lambda: mixed.Split(prec, self.Wr, self.ring_b, nslice))
-            timed("gemm_ring", lambda: mixed.gemm_split(ra, rb, R, alpha=1.0, cview=self.ring_c,
-                                                        max_level=lev_ring))
-            del ra, rb
+            if "ladder" in parts:
+                sa = timed("split_integrals", lambda: mixed.Split(prec, self.Vvvvv, self.lad_a, nslice))
+                sb = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.lad_b, nslice))
+                timed("gemm_ladder", lambda: mixed.gemm_split(sa, sb, parts["ladder"], alpha=0.5,
+                                                              ldc=o * o, max_level=lev_lad))
+                del sa, sb
+            if "ring" in parts:
+                ra = timed("split_amplitudes", lambda: mixed.Split(prec, self.T, self.ring_a, nslice))
+                rb = timed("split_integrals", lambda: mixed.Split(prec, self.Wr, self.ring_b, nslice))
+                timed("gemm_ring", lambda: mixed.gemm_split(ra, rb, parts["ring"], alpha=1.0,
+                                                            cview=self.ring_c, max_level=lev_ring))
+                del ra, rb
+        R = self.Vvvoo.clone()
+        for x in parts.values():
+            R += x
         Tn = ops.denom_update(R, self.eo, self.ev)
         E = ops.energy(self.Voovv, Tn, o, v).item()
-        return R, Tn, E, ph
+        return R, Tn, E, ph, parts
+
+    def energy_of(self, X):
+        """1/4 sum V_oovv[i,j,a,b] X[a,b,i,j] / D_ijab: the energy carried by
+        one contribution X to the residual (E is linear in R)."""
+        return ops.energy(self.Voovv, ops.denom_update(X, self.eo, self.ev), self.o, self.v).item()
 
 
 def study(o, v, variants=VARIANTS, reps=1, log=print):
@@ -135,10 +151,11 @@ def study(o, v, variants=VARIANTS, reps=1, log=print):
     out = {"o": o, "v": v, "flops_ladder": sysm.lad_flops, "flops_ring": sysm.ring_flops,
            "variants": []}
     R64 = E64 = t64 = None
+    P64 = {}
     for name, prec, ns, lev in variants:
         best = None
         for _ in range(max(1, reps)):
-            R, Tn, E, ph = sysm.iterate(prec, ns, lev)
+            R, Tn, E, ph, parts = sysm.iterate(prec, ns, lev)
             tc = sum(ph.values())
             if best is None or tc < best[0]:
                 best = (tc, ph)
@@ -149,8 +166,10 @@ def study(o, v, variants=VARIANTS, reps=1, log=print):
                    tflops=flops / tc / 1e12, energy=E)
         if prec is None:
             R64, E64, t64 = R, E, tc
+            P64 = {k: (x, sysm.energy_of(x)) for k, x in parts.items()}
             rec.update(max_abs_err=0.0, max_rel_err=0.0, energy_err=0.0, speedup=1.0,
                        speedup_integrals_resident=1.0, products=1)
+            rec["terms"] = {k: {"energy": e} for k, (x, e) in P64.items()}
         else:
             d = (R - R64).abs().max().item()
             rec.update(max_abs_err=d, max_rel_err=d / R64.abs().max().item(),
@@ -158,9 +177,15 @@ def study(o, v, variants=VARIANTS, reps=1, log=print):
                        speedup_integrals_resident=t64 / tcached,
                        products=(mixed.n_products(prec, ns, lev) if not isinstance(lev, tuple)
                                  else [mixed.n_products(prec, ns, x) for x in lev]))
+            rec["terms"] = {}
+            for k, x in parts.items():
+                x64, e64 = P64[k]
+                dk = (x - x64).abs().max().item()
+                rec["terms"][k] = {"max_rel_err": dk / x64.abs().max().item(),
+                                   "energy_err": abs(sysm.energy_of(x) - e64)}
         out["variants"].append(rec)
         log(json.dumps(rec))
-        del R, Tn
+        del R, Tn, parts
         torch.cuda.empty_cache()
     return out
 

@@ -77,7 +77,6 @@ def lib() -> C.CDLL:
     L.orc_energy.restype = C.c_double
     L.orc_norm2.argtypes = [_dp, _i64]
     L.orc_norm2.restype = C.c_double
-    L.orc_block_pairs.argtypes = [_i64, _lp, _lp, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp]
     _LIB = L
     return L
 
@@ -251,10 +250,22 @@ def norm2(R):
     return float(lib().orc_norm2(R, R.size))
 
 
-def block_pairs(m, n, k, aoff, boff, coff, alpha, A, B, Cbuf):
-    lib().orc_block_pairs(len(m), _np64(m), _np64(n), _np64(k), _np64(aoff), _np64(boff),
-                          _np64(coff), _f64(alpha), _f64(A).ravel(), _f64(B).ravel(), Cbuf)
-    return Cbuf
+def ccd_iteration(Vvvvv, Vvvoo, W, T, eo, ev):
+    """One CCD-like amplitude iteration of the precision study, numpy einsum
+    over dense arrays: ladder[a,b,i,j] = 1/2 sum_cd V[a,b,c,d] T[c,d,i,j],
+    ring[a,b,i,j] = sum_kc T[a,c,i,k] W[k,j,c,b], R = V_vvoo + ladder + ring,
+    T_new = R / D (``denom_update``) and E = 1/4 sum V_oovv[i,j,a,b]
+    T_new[a,b,i,j] with V_oovv[i,j,a,b] = V_vvoo[a,b,i,j] (``energy``).
+    Returns (ladder, ring, T_new, E), the first three as [v,v,o,o] arrays.
+    New (the reference stops at the contraction); pinned by brute-force
+    loops on a tiny system (tests/test_oracle.py)."""
+    v, o = Vvvoo.shape[0], Vvvoo.shape[2]
+    lad = 0.5 * np.einsum("abcd,cdij->abij", Vvvvv, T, optimize=True)
+    ring = np.einsum("acik,kjcb->abij", T, W, optimize=True)
+    R = Vvvoo + lad + ring
+    Tn = denom_update(R, eo, ev).reshape(v, v, o, o)
+    E = energy(np.ascontiguousarray(np.transpose(Vvvoo, (2, 3, 0, 1))), Tn, o, v)
+    return lad, ring, Tn, E
 
 
 # --------------------------------------------------------------------------
@@ -509,7 +520,7 @@ def _letters(names):
     return table
 
 
-def termlist_statement(kind, c_idx, terms, arrays):
+def termlist_statement(kind, c_idx, terms, arrays, optimize=False):
     """One statement of a term list.  kind "mul": terms = [(name, idx),
     (name, idx)], result C[c_idx] = sum over the shared indices of A * B (the
     DLTC multiplication, DLTC/expr.cc:13-67 with DLTC/compute.cc:3-36, as a
@@ -521,7 +532,7 @@ def termlist_statement(kind, c_idx, terms, arrays):
         L = _letters(list(c_idx) + list(ai) + list(bi))
         spec = "".join(L[x] for x in ai) + "," + "".join(L[x] for x in bi) + "->" + \
             "".join(L[x] for x in c_idx)
-        return np.einsum(spec, arrays[an], arrays[bn], optimize=False)
+        return np.einsum(spec, arrays[an], arrays[bn], optimize=optimize)
     out = None
     for name, idx, coef in terms:
         perm = [list(idx).index(x) for x in c_idx]
@@ -537,3 +548,80 @@ def termlist_run(statements, arrays):
     for lhs, kind, c_idx, terms in statements:
         arrays[lhs] = termlist_statement(kind, c_idx, terms, arrays)
     return arrays
+
+
+def termlist_run_sampled(statements, get_input, shapes, want):
+    """The statements of ``termlist_run`` evaluated only on boxes of index
+    values: ``want`` maps an output name to one list of index values per
+    dimension (None = the whole range).  A backward pass gives every array
+    the box its readers need (a contracted dimension needs its whole range,
+    an external or summed one the values of the result's box; boxes of one
+    array are united per dimension), a forward pass evaluates each statement
+    on its box with the same Einstein sums as ``termlist_statement``.
+    ``get_input(name)`` returns a whole input array, ``shapes`` maps every
+    array name to its extents.  Returns {output name: array over its box}.
+    Pinned in tests against ``termlist_run`` on whole small systems."""
+    def full(n, d):
+        return np.arange(shapes[n][d])
+
+    need = {}
+
+    def add(name, vals):
+        cur = need.get(name)
+        if cur is None:
+            need[name] = [np.unique(v) for v in vals]
+        else:
+            need[name] = [np.union1d(a, b) for a, b in zip(cur, vals)]
+
+    for name, box in want.items():
+        add(name, [full(name, d) if v is None else np.asarray(v) for d, v in enumerate(box)])
+    lhs_seen = set()
+    for lhs, kind, c_idx, terms in reversed(statements):
+        assert lhs not in lhs_seen, f"{lhs} assigned twice"
+        lhs_seen.add(lhs)
+        if lhs not in need:
+            continue
+        box = dict(zip(c_idx, need[lhs]))
+        for t in terms:
+            tname, tidx = t[0], t[1]
+            add(tname, [box[x] if x in box else full(tname, d) for d, x in enumerate(tidx)])
+
+    store = {}  # name -> (values per dim, array over the box)
+
+    def take(name, idx, box):
+        vals, arr = store[name]
+        pos = []
+        for d, x in enumerate(idx):
+            want_v = box.get(x, vals[d])
+            p = np.searchsorted(vals[d], want_v)
+            assert np.array_equal(vals[d][p], want_v), (name, d)
+            pos.append(p)
+        return arr[np.ix_(*pos)]
+
+    for lhs, kind, c_idx, terms in statements:
+        for t in terms:
+            if t[0] not in store and t[0] in need:
+                vals = need[t[0]]
+                store[t[0]] = (vals, np.ascontiguousarray(get_input(t[0])[np.ix_(*vals)]))
+        if lhs not in need:
+            continue
+        vals = need[lhs]
+        box = dict(zip(c_idx, vals))
+        if kind == "mul":
+            (an, ai), (bn, bi) = terms
+            contr = [x for x in ai if x in bi and x not in c_idx]
+            bx = {x: v for x, v in box.items()}
+            for x in contr:
+                bx.pop(x, None)
+            sub = {an: take(an, ai, bx), bn: take(bn, bi, bx)}
+            out = termlist_statement("mul", c_idx, [(an, ai), (bn, bi)], sub, optimize=True)
+        else:
+            sub = {}
+            tt = []
+            for k, (name, idx, coef) in enumerate(terms):
+                key = f"{name}#{k}"
+                sub[key] = take(name, idx, box)
+                tt.append((key, idx, coef))
+            out = termlist_statement("sum", c_idx, tt, sub)
+        store[lhs] = (vals, out)
+    return {n: store[n][1] for n in want}

@@ -514,15 +514,3 @@ double orc_norm2(const double *R, int64_t n)
     return (double)sqrtl(s);
 }
 
-/* Generic dense block-pair contraction used to check the grouped launch:
- * for each pair p: C[cb[p]] (m x n, ldc) += alpha * A[ab[p]] (m x k) * B[bb[p]] (k x n)
- * with operand offsets in elements.  Sequential, pair order. */
-void orc_block_pairs(int64_t npairs, const int64_t *m, const int64_t *n,
-                     const int64_t *k, const int64_t *aoff, const int64_t *boff,
-                     const int64_t *coff, const double *alpha, const double *A,
-                     const double *B, double *C)
-{
-    for (int64_t p = 0; p < npairs; p++)
-        orc_dgemm(0, 0, m[p], n[p], k[p], alpha[p], A + aoff[p], k[p],
-                  B + boff[p], n[p], C + coff[p], n[p]);
-}

@@ -218,7 +218,14 @@ def test_float_split_bitwise_vs_host(dev, prec, rows, K):
     A = rng.standard_normal((rows, K)) * np.exp2(rng.integers(-8, 8, size=(rows, 1)))
     A[1] = 0.0
     sp = mixed.Split(prec, torch.from_numpy(A).to(dev), mixed.dense_view(rows, K))
+    # the same matrix stored column-major (rows contiguous: the row-run
+    # launches at 40 x 20000) gives the same planes bit for bit
+    At = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
+    cm = mixed.Split(prec, At, ops.View(rows, K, np.arange(rows), np.arange(K) * rows))
     torch.cuda.synchronize()
+    assert torch.equal(cm.buf, sp.buf)
+    if sp.rexp is not None:
+        assert torch.equal(cm.rexp, sp.rexp)
     rp, kp = (rows + 127) // 128 * 128, (K + 127) // 128 * 128
     buf = sp.buf.cpu().numpy()
     Ah = torch.zeros((rp, kp), dtype=torch.float64)

@@ -229,6 +229,35 @@ def test_cc_closed_forms():
     assert abs(oracle.norm2(R) - np.linalg.norm(R)) <= 1e-13 * np.linalg.norm(R)
 
 
+def test_ccd_iteration_vs_loops():
+    """The precision-study iteration: ladder and ring by brute-force loops,
+    then T_new = R / D and E = 1/4 sum V_oovv T_new element by element."""
+    o, v = 2, 3
+    rng = np.random.default_rng(12)
+    V = rng.standard_normal((v, v, v, v))
+    Vvvoo = rng.standard_normal((v, v, o, o))
+    W = rng.standard_normal((o, o, v, v))
+    T = rng.standard_normal((v, v, o, o))
+    eo = np.array([-1.5, -0.8])
+    ev = np.array([0.3, 1.1, 2.6])
+    lad, ring, Tn, E = oracle.ccd_iteration(V, Vvvoo, W, T, eo, ev)
+    L = np.zeros((v, v, o, o))
+    Rg = np.zeros((v, v, o, o))
+    for a, b, i, j in itertools.product(range(v), range(v), range(o), range(o)):
+        for c, d in itertools.product(range(v), range(v)):
+            L[a, b, i, j] += 0.5 * V[a, b, c, d] * T[c, d, i, j]
+        for k, c in itertools.product(range(o), range(v)):
+            Rg[a, b, i, j] += T[a, c, i, k] * W[k, j, c, b]
+    np.testing.assert_allclose(lad, L, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(ring, Rg, rtol=1e-13, atol=1e-13)
+    Eref = 0.0
+    for a, b, i, j in itertools.product(range(v), range(v), range(o), range(o)):
+        t = (Vvvoo[a, b, i, j] + L[a, b, i, j] + Rg[a, b, i, j]) / (eo[i] + eo[j] - ev[a] - ev[b])
+        assert abs(Tn[a, b, i, j] - t) <= 1e-13 * max(1.0, abs(t))
+        Eref += 0.25 * Vvvoo[a, b, i, j] * t
+    assert abs(E - Eref) <= 1e-13 * max(1.0, abs(Eref))
+
+
 def test_permutation_maps_c1_c2():
     # C2: contract("i,k,a,c", "k,j,c,b", "i,j,a,b")
     br = dict(i=12, j=12, k=12, a=48, b=48, c=48)

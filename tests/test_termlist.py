@@ -104,3 +104,41 @@ def test_einsum_oracle_against_loops():
     for a, b, i, j in itertools.product(range(3), range(4), range(2), range(5)):
         ref[a, b, i, j] = P[a, b, i, j] - 0.5 * Q[b, a, j, i]
     np.testing.assert_allclose(got, ref, rtol=1e-14, atol=1e-14)
+
+
+def _stm(tl):
+    out = []
+    for s in tl.statements:
+        if s.kind == "mul":
+            out.append((s.lhs.name, "mul", s.lhs.idx, [(t.name, t.idx) for t in s.terms]))
+        else:
+            out.append((s.lhs.name, "sum", s.lhs.idx, [(t.name, t.idx, t.coef) for t in s.terms]))
+    return out
+
+
+@pytest.mark.parametrize("which", ["t1", "t2"])
+def test_sampled_oracle_equals_whole_run(lists, which):
+    """termlist_run_sampled on boxes of the residuals equals termlist_run of
+    the whole system at those elements (asymmetric extents, several boxes,
+    one of them whole)."""
+    tl = lists[which]
+    sizes = {"Oa": 4, "Ob": 3, "Va": 7, "Vb": 6}
+    shapes = {n: tuple(sizes[r] for r in a.ranges) for n, a in tl.arrays.items()}
+    rng = np.random.default_rng(5)
+    inputs = {n: rng.standard_normal(shapes[n]) for n in tl.inputs()}
+    ref = oracle.termlist_run(_stm(tl), dict(inputs))
+    for trial in range(3):
+        want = {}
+        for n in tl.outputs():
+            shp = shapes[n]
+            if trial == 2:
+                want[n] = [None] * len(shp)
+            else:
+                want[n] = [sorted(rng.choice(shp[0], 2, replace=False).tolist())] + \
+                    [None if d != 1 else [int(rng.integers(shp[1]))] for d in range(1, len(shp))]
+        got = oracle.termlist_run_sampled(_stm(tl), inputs.__getitem__, shapes, want)
+        for n, box in want.items():
+            idx = [np.arange(shapes[n][d]) if b is None else np.asarray(b) for d, b in enumerate(box)]
+            r = ref[n][np.ix_(*idx)]
+            assert got[n].shape == r.shape
+            assert np.max(np.abs(got[n] - r)) <= 1e-13 * np.max(np.abs(r)), (which, n, trial)
