@@ -1,0 +1,214 @@
+"""UHF CCSD iterations on one GPU: the T1 and T2 residual programs of the
+UHF term lists (``_uhf_t1.eq``, ``_uhf_t2.eq``) served from device-resident
+tensors, then the FP64 amplitude update, the correlation energy and the
+residual norms (``dtcf_uhf_ccsd_step_f64``).  The reference stops at the
+residuals (ra_vo / rb_vo, _uhf_t1.eq:143,242; rab_vvoo / raa_vvoo /
+rbb_vvoo, _uhf_t2.eq:519,769,996); the loop around them replaces the
+layered per-iteration drivers of DLTC (DLTC/ccsd-t2.cc, ccd-t2.cc:97-131).
+
+One iteration: both residual programs are submitted as requests (they run
+concurrently on the server's streams), then
+    t_ai   += r_ai   / (e_i - e_a)
+    t_abij += r_abij / (((e_i + e_j) - e_a) - e_b)
+per spin block, and E(T_new).  The residual programs may run a mixed
+precision (digit slices of INT8 MFMA products, recombined in FP64); the
+update, the energy and the norms are FP64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import mixed, serve, termlist
+from ._lib import check, lib
+
+AMPS = ("ta_vo", "tb_vo", "taa_vvoo", "tab_vvoo", "tbb_vvoo")
+RESID = {"ta_vo": "ra_vo", "tb_vo": "rb_vo", "taa_vvoo": "raa_vvoo", "tab_vvoo": "rab_vvoo",
+         "tbb_vvoo": "rbb_vvoo"}
+OUT_NAMES = ("E", "E1", "Eaa", "Eab", "Ebb", "r1", "r2", "dt")
+
+
+class UHFDesc(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("Oa", "Ob", "Va", "Vb")] + \
+        [(n, C.c_void_p) for n in ("eoa", "eob", "eva", "evb", "fa_ov", "fb_ov", "vaa_oovv",
+                                   "vab_oovv", "vbb_oovv", "ra", "rb", "raa", "rab", "rbb",
+                                   "ta", "tb", "taa", "tab", "tbb")]
+
+
+def _declare():
+    L = lib()
+    L.dtcf_uhf_ccsd_step_f64.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.dtcf_uhf_ccsd_step_f64.restype = C.c_int
+
+
+_declare()
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def step(desc: UHFDesc, update: bool, out: torch.Tensor):
+    """Enqueues the update (or, update False, only the energy) on torch's
+    current stream; ``out`` (8 float64 on the device) receives
+    E, E1, Eaa, Eab, Ebb, |R1|, |R2|, |dT|."""
+    check(lib().dtcf_uhf_ccsd_step_f64(C.byref(desc), 1 if update else 0,
+                                       C.c_void_p(out.data_ptr()), _stream()))
+    return out
+
+
+def make_desc(sizes, ptr, res=None):
+    """Descriptor over device pointers: ``ptr`` maps eoa, eob, eva, evb,
+    fa_ov, fb_ov, vaa_oovv, vab_oovv, vbb_oovv and the amplitudes to
+    addresses, ``res`` the residual names (ra_vo, ...) to addresses."""
+    d = UHFDesc()
+    d.Oa, d.Ob, d.Va, d.Vb = (int(sizes[k]) for k in ("Oa", "Ob", "Va", "Vb"))
+    for k in ("eoa", "eob", "eva", "evb", "fa_ov", "fb_ov", "vaa_oovv", "vab_oovv", "vbb_oovv"):
+        setattr(d, k, ptr[k])
+    for amp, short in zip(AMPS, ("ta", "tb", "taa", "tab", "tbb")):
+        setattr(d, short, ptr[amp])
+    if res is not None:
+        for amp, short in zip(AMPS, ("ra", "rb", "raa", "rab", "rbb")):
+            setattr(d, short, res[RESID[amp]])
+    return d
+
+
+class CCSystem(serve.System):
+    """A synthetic UHF system for CC iterations, resident in a server arena.
+
+    Orbital energies as the precision study draws them: occupied
+    U[-2, -0.5], virtual U[0.2, 3], separately per spin.  The Fock blocks
+    f_oo, f_vv hold them on the diagonal plus a symmetric off-diagonal part
+    N(0, fock_sigma^2); f_ov is N(0, fock_sigma^2) and f_vo its transpose.
+    Integrals are N(0, sigma^2), antisymmetrised in every index pair their
+    declaration names (each pair (X - X^pq) / sqrt 2, which keeps sigma),
+    with v_vvoo[a,b,i,j] = v_oovv[i,j,a,b] (real orbitals).  Amplitudes start
+    at zero, so the first iteration gives t_ai = f_ai / D_ai and
+    t_abij = v_abij / D_abij."""
+
+    def __init__(self, server, o, v, seed=7, sigma=0.05, fock_sigma=0.01, sizes=None, lists=None):
+        sizes = sizes or serve.system_sizes(o, v)
+        self.sigma, self.fock_sigma = sigma, fock_sigma
+        self.eps = {
+            "eoa": mixed.fill_uniform(sizes["Oa"], seed * 8 + 1, -2.0, -0.5),
+            "eob": mixed.fill_uniform(sizes["Ob"], seed * 8 + 2, -2.0, -0.5),
+            "eva": mixed.fill_uniform(sizes["Va"], seed * 8 + 3, 0.2, 3.0),
+            "evb": mixed.fill_uniform(sizes["Vb"], seed * 8 + 4, 0.2, 3.0),
+        }
+        lists = lists or [termlist.load("t1"), termlist.load("t2")]
+        super().__init__(server, o, v, lists, seed=seed, sizes=sizes)
+
+    def _normal(self, name, shp, sigma):
+        return mixed.fill_normal(int(np.prod(shp)), self.name_seed(name), sigma).view(*shp)
+
+    def make(self, name, decl, shp):
+        spin = name[1]
+        if name.startswith("t"):
+            return torch.zeros(int(np.prod(shp)), dtype=torch.float64, device="cuda")
+        if name.startswith("f"):
+            block = name.split("_")[1]
+            if block in ("oo", "vv"):
+                eps = self.eps[("eo" if block == "oo" else "ev") + spin]
+                n = shp[0]
+                N = self._normal(name, shp, 1.0)
+                N = (N + N.t()) / math.sqrt(2.0)
+                N.fill_diagonal_(0.0)
+                return (torch.diag(eps) + self.fock_sigma * N).reshape(-1).contiguous()
+            if block == "ov":
+                return self._normal(name, shp, self.fock_sigma).reshape(-1)
+            # f_vo = f_ov^T
+            ov = self._normal(f"f{spin}_ov", (shp[1], shp[0]), self.fock_sigma)
+            return ov.t().contiguous().reshape(-1)
+        if name.endswith("_vvoo"):
+            src = name[:-5] + "_oovv"  # v_vvoo[a,b,i,j] = v_oovv[i,j,a,b]
+            oovv = self.make(src, self.decl.get(src) or _oovv_decl(name), (shp[2], shp[3], shp[0], shp[1]))
+            return oovv.view(shp[2], shp[3], shp[0], shp[1]).permute(2, 3, 0, 1).contiguous().reshape(-1)
+        X = self._normal(name, shp, self.sigma)
+        for g in decl.groups:
+            p, q = g
+            assert shp[p] == shp[q]
+            X = (X - X.transpose(p, q)) / math.sqrt(2.0)
+        return X.contiguous().reshape(-1)
+
+
+def _oovv_decl(vvoo_name):
+    """Declaration of v*_oovv when only v*_vvoo is an input of the lists:
+    same-spin blocks are antisymmetric in both pairs."""
+    same = vvoo_name[1] == vvoo_name[2]
+    return termlist.Array(vvoo_name[:-5] + "_oovv", (), 2, ((0, 1), (2, 3)) if same else ())
+
+
+class UHFCCSD:
+    """CCSD iterations over a ``CCSystem``.  prec 0: FP64 residual programs;
+    otherwise a mixed precision for the contractions that take it
+    (``serve.Program``), the integral inputs split once and kept resident
+    when ``resident_integrals``."""
+
+    def __init__(self, server: serve.Server, system: CCSystem, prec=0, nslice=5, max_level=0,
+                 mixed_min=None, resident_integrals=True):
+        self.server, self.system = server, system
+        self.t1, self.t2 = termlist.load("t1"), termlist.load("t2")
+        self.progs = []
+        for tl in (self.t1, self.t2):
+            res = serve.integral_inputs(tl) if (prec and resident_integrals) else ()
+            self.progs.append(serve.Program(server, tl, system, prec=prec, nslice=nslice,
+                                            max_level=max_level, mixed_min=mixed_min,
+                                            resident=res))
+        self.res = {}
+        self.outs = []
+        for p in self.progs:
+            bufs = p.new_outputs()
+            self.outs.append(bufs)
+            self.res.update(dict(zip(p.outputs, bufs)))
+        ptr = dict(system.ptr)
+        ptr.update({k: t.data_ptr() for k, t in system.eps.items()})
+        self.desc = make_desc(system.sizes, ptr, {k: t.data_ptr() for k, t in self.res.items()})
+        self.out = torch.zeros(8, dtype=torch.float64, device="cuda")
+        self.flops = sum(p.flops for p in self.progs)
+
+    def residuals(self):
+        """Evaluates R1 and R2 at the current amplitudes (both programs at
+        once); returns the wall time in seconds."""
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rids = [p.submit(outs)[0] for p, outs in zip(self.progs, self.outs)]
+        for rid in rids:
+            check(self.server.wait(rid, 600000.0))
+        dt = time.perf_counter() - t
+        for rid in rids:
+            self.server.release(rid)
+        return dt
+
+    def step(self, update=True):
+        """Update (or only evaluate the energy) and return the outputs."""
+        step(self.desc, update, self.out)
+        vals = self.out.cpu().numpy()  # synchronises: the next requests read the new amplitudes
+        return dict(zip(OUT_NAMES, (float(x) for x in vals)))
+
+    def iterate(self):
+        t = time.perf_counter()
+        tr = self.residuals()
+        rec = self.step(True)
+        rec["seconds"] = time.perf_counter() - t
+        rec["seconds_residuals"] = tr
+        rec["tflops_residuals"] = self.flops / tr / 1e12
+        return rec
+
+    def energy(self):
+        return self.step(False)
+
+    def amplitudes(self):
+        return {n: self.system.host(n) for n in AMPS}
+
+    def residual_arrays(self):
+        return {n: t.cpu().numpy().reshape(s) for p, outs in zip(self.progs, self.outs)
+                for n, t, s in zip(p.outputs, outs, p.out_shapes)}
+
+    def destroy(self):
+        for p in self.progs:
+            p.destroy()
+        self.progs = []

@@ -195,6 +195,7 @@ struct PArr {
     int64_t elems = 1;
     int role = DTCF_ARR_TEMP;
     const double *bound = nullptr;
+    bool resident = false;  // inputs: slices made once at finalize (mixed contractions)
     int64_t off = -1;       // temporaries: offset in the request block
     int producer = -1, last_use = -1, out_index = -1;
     std::vector<int> readers;  // statements reading the array, in program order
@@ -237,8 +238,8 @@ struct dtcf_program_s {
     int prec = 0, nslice = 0, max_level = 0;
     // mixed precision only for contractions with m n / (m + n) >= mixed_min
     double mixed_min = 256.0;
-    // slices of input operands made once at finalize and kept (server arena)
-    bool resident_slices = false;
+    // slices of the inputs marked resident, made once at finalize and kept
+    // (server arena)
     std::vector<void *> res_allocs;
     std::vector<PArr> arrs;
     std::map<std::string, int> byname;
@@ -476,7 +477,7 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
         // GEMM operand 0 is array b when swapped, a otherwise
         const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
         for (int q = 0; q < 2; q++)
-            s.res[q] = p->resident_slices && p->arrs[arr[q]].role == DTCF_ARR_INPUT;
+            s.res[q] = p->arrs[arr[q]].role == DTCF_ARR_INPUT && p->arrs[arr[q]].resident;
         int64_t off = 0;
         if (!s.res[0]) { s.a_off = off; off += s.sa_bytes; }
         if (!s.res[1]) { s.b_off = off; off += s.sb_bytes; }
@@ -847,7 +848,27 @@ extern "C" int dtcf_program_bind(dtcf_program p, const char *name, const double 
     DTCF_REQUIRE(i >= 0, "program_bind: unknown array");
     DTCF_REQUIRE(p->arrs[i].role == DTCF_ARR_INPUT, "program_bind: only inputs are bound");
     DTCF_REQUIRE(ptr, "program_bind: null pointer");
+    if (p->final) {
+        // Requests read bound inputs when their statements issue; resident
+        // slices were cut from the pointer bound at finalize.
+        DTCF_REQUIRE(!p->arrs[i].resident,
+                     "program_bind: an input with resident slices cannot be rebound after finalize");
+        std::lock_guard<std::mutex> lk(p->srv->mu);
+        DTCF_REQUIRE(p->refs == 0, "program_bind: requests of this program are in flight");
+        p->arrs[i].bound = ptr;
+        return DTCF_OK;
+    }
     p->arrs[i].bound = ptr;
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_program_set_resident(dtcf_program p, const char *name, int on)
+{
+    DTCF_REQUIRE(p && !p->final && p->st.empty(), "program: set_resident before any statement");
+    const int i = arr_index(p, name);
+    DTCF_REQUIRE(i >= 0, "program_set_resident: unknown array");
+    DTCF_REQUIRE(p->arrs[i].role == DTCF_ARR_INPUT, "program_set_resident: only inputs are resident");
+    p->arrs[i].resident = on != 0;
     return DTCF_OK;
 }
 
@@ -1088,13 +1109,6 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
     }
     if (!p->res_allocs.empty()) DTCF_HIP(hipStreamSynchronize(nullptr));
     p->final = true;
-    return DTCF_OK;
-}
-
-extern "C" int dtcf_program_set_resident_slices(dtcf_program p, int on)
-{
-    DTCF_REQUIRE(p && !p->final && p->st.empty(), "program: set_resident_slices before any statement");
-    p->resident_slices = on != 0;
     return DTCF_OK;
 }
 

@@ -49,7 +49,7 @@ def _declare():
         "dtcf_program_finalize": [vp],
         "dtcf_program_set_layout": [vp, C.c_int],
         "dtcf_program_set_mixed_min": [vp, C.c_double],
-        "dtcf_program_set_resident_slices": [vp, C.c_int],
+        "dtcf_program_set_resident": [vp, C.c_char_p, C.c_int],
         "dtcf_program_get_layout": [vp, C.POINTER(C.c_int)],
         "dtcf_program_info": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(dbl), C.POINTER(i64)],
         "dtcf_program_destroy": [vp],
@@ -174,17 +174,13 @@ class System:
         decl = {}
         for tl in lists:
             decl.update({n: tl.arrays[n] for n in names if n in tl.arrays})
+        self.decl = decl
         for n in names:
             a = decl[n]
             shp = tuple(self.sizes[r] for r in a.ranges)
             cnt = int(np.prod(shp))
             p = server.alloc(cnt)
-            s = (seed * 1000003 + zlib.crc32(n.encode())) & 0xFFFFFFFF
-            raw = mixed.fill_normal(cnt, s, 0.05)
-            anti = len(shp) == 4 and set(a.groups) == {(0, 1), (2, 3)} and \
-                shp[0] == shp[1] and shp[2] == shp[3]
-            if anti:
-                raw = mixed.antisymmetrize4(raw, shp)
+            raw = self.make(n, a, shp)
             torch.cuda.synchronize()
             st = _hip().hipMemcpy(C.c_void_p(p), C.c_void_p(raw.data_ptr()), cnt * 8, 3)
             if st != 0:
@@ -192,6 +188,19 @@ class System:
             del raw
             self.ptr[n], self.shape[n] = p, shp
         torch.cuda.synchronize()
+
+    def name_seed(self, name):
+        return (self.seed * 1000003 + zlib.crc32(name.encode())) & 0xFFFFFFFF
+
+    def make(self, name, decl, shp):
+        """Contents of input ``name`` (a flat float64 device tensor)."""
+        cnt = int(np.prod(shp))
+        raw = mixed.fill_normal(cnt, self.name_seed(name), 0.05)
+        anti = len(shp) == 4 and set(decl.groups) == {(0, 1), (2, 3)} and \
+            shp[0] == shp[1] and shp[2] == shp[3]
+        if anti:
+            raw = mixed.antisymmetrize4(raw, shp)
+        return raw
 
     def host(self, name):
         return to_host(self.ptr[name], int(np.prod(self.shape[name]))).reshape(self.shape[name])
@@ -203,14 +212,22 @@ class System:
 LAYOUT_AUTO, LAYOUT_REUSE, LAYOUT_SEPARATE = 0, 1, 2
 
 
+def integral_inputs(tl: termlist.TermList):
+    """Inputs of a term list that stay the same in every CC iteration: the
+    integrals and Fock blocks (every input but the amplitudes t*)."""
+    return [n for n in tl.inputs() if not n.startswith("t")]
+
+
 class Program:
     """One term list planned over a system.  ``expose`` makes every assigned
     array an output (for parity tests of each statement); ``layout`` picks the
-    placement of temporaries (LAYOUT_*; ``self.layout`` is the one used)."""
+    placement of temporaries (LAYOUT_*; ``self.layout`` is the one used);
+    ``resident`` names inputs whose mixed-precision slices are made once at
+    finalize (inputs that do not change between requests, e.g.
+    ``integral_inputs(tl)``)."""
 
     def __init__(self, server: Server, tl: termlist.TermList, system: System, prec=0, nslice=5,
-                 max_level=0, expose=False, layout=LAYOUT_AUTO, mixed_min=None,
-                 resident_slices=False):
+                 max_level=0, expose=False, layout=LAYOUT_AUTO, mixed_min=None, resident=()):
         self.server, self.tl, self.system = server, tl, system
         L = lib()
         h = C.c_void_p()
@@ -218,8 +235,6 @@ class Program:
         self.h = h
         if mixed_min is not None:
             check(L.dtcf_program_set_mixed_min(h, float(mixed_min)))
-        if resident_slices:
-            check(L.dtcf_program_set_resident_slices(h, 1))
         self.prec = prec
         used = []
         for s in tl.statements:
@@ -238,6 +253,8 @@ class Program:
                 check(L.dtcf_program_array(h, n.encode(), len(shp), dims, role))
                 if role == ROLE_INPUT:
                     check(L.dtcf_program_bind(h, n.encode(), C.c_void_p(system.ptr[n])))
+                    if n in resident:
+                        check(L.dtcf_program_set_resident(h, n.encode(), 1))
                 elif role == ROLE_OUTPUT:
                     self.outputs.append(n)
                     self.out_shapes.append(shp)

@@ -24,7 +24,7 @@ SYSTEMS = ((32, 128), (64, 256), (96, 384))
 
 
 def build(budget_gb=200.0, nslots=8, systems=SYSTEMS, precs=((0, 0),), log=print,
-          resident_slices=False):
+          resident_integrals=False):
     t1, t2 = termlist.load("t1"), termlist.load("t2")
     srv = serve.Server(int(budget_gb * (1 << 30)), nslots)
     progs = []
@@ -35,8 +35,8 @@ def build(budget_gb=200.0, nslots=8, systems=SYSTEMS, precs=((0, 0),), log=print
             prec, ns = pr[0], pr[1]
             lev = pr[2] if len(pr) > 2 else 0
             for which, tl in (("t1", t1), ("t2", t2)):
-                p = serve.Program(srv, tl, sysm, prec=prec, nslice=ns, max_level=lev,
-                                  resident_slices=resident_slices and prec != 0)
+                res = serve.integral_inputs(tl) if resident_integrals and prec != 0 else ()
+                p = serve.Program(srv, tl, sysm, prec=prec, nslice=ns, max_level=lev, resident=res)
                 p.label = f"{which}/o{o}v{v}/" + ("fp64" if prec == 0 else
                                                    mixed.NAMES[prec] + f"x{ns}" + (f"L{lev}" if lev else ""))
                 progs.append(p)
@@ -119,8 +119,9 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="add INT8x5 programs to the mix")
     ap.add_argument("--int8x4l6", action="store_true",
                     help="with --mixed: four digits, levels up to 6, instead of INT8x5")
-    ap.add_argument("--resident-slices", action="store_true",
-                    help="mixed programs split their input operands once, at finalize")
+    ap.add_argument("--resident-integrals", action="store_true",
+                    help="mixed programs split their integral inputs once, at finalize "
+                         "(the amplitudes are split by every request)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     check(lib().dtcf_init(0))
@@ -128,7 +129,7 @@ def main():
              else ((0, 0),))
     log = lambda s: print(s, flush=True)
     srv, progs = build(args.budget_gb, args.slots, precs=precs, log=log,
-                       resident_slices=args.resident_slices)
+                       resident_integrals=args.resident_integrals)
     alone = service_times(srv, progs)
     for k, v in alone.items():
         log(f"alone {k}: {v['seconds'] * 1e3:.1f} ms, {v['tflops']:.1f} TF/s, block {v['block_gb']:.2f} GB")

@@ -412,6 +412,45 @@ int dtcf_energy_f64(const double *V, const double *T, int64_t o, int64_t v, doub
 int dtcf_norm2_f64(const double *R, int64_t n, double *out_dev, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* 7b. UHF CCSD amplitude update and correlation energy (new; the      */
+/*     reference stops at the residuals ra_vo / rb_vo, _uhf_t1.eq:143, */
+/*     242, and rab_vvoo / raa_vvoo / rbb_vvoo, _uhf_t2.eq:519,769,996)  */
+/* ------------------------------------------------------------------ */
+/* Dense row-major device arrays of one UHF system, the names and layouts
+ * of the term lists: orbital energies eoa[Oa], eob[Ob], eva[Va], evb[Vb]
+ * (the Fock diagonals), fa_ov[Oa,Va], fb_ov[Ob,Vb], vaa_oovv[Oa,Oa,Va,Va],
+ * vab_oovv[Oa,Ob,Va,Vb], vbb_oovv[Ob,Ob,Vb,Vb], amplitudes ta[Va,Oa],
+ * tb[Vb,Ob], taa[Va,Va,Oa,Oa], tab[Va,Vb,Oa,Ob], tbb[Vb,Vb,Ob,Ob] and
+ * residuals of the same shapes. */
+typedef struct {
+    int64_t Oa, Ob, Va, Vb;
+    const double *eoa, *eob, *eva, *evb;
+    const double *fa_ov, *fb_ov;
+    const double *vaa_oovv, *vab_oovv, *vbb_oovv;
+    const double *ra, *rb, *raa, *rab, *rbb;
+    double *ta, *tb, *taa, *tab, *tbb;
+} dtcf_uhf_ccsd;
+
+enum {
+    DTCF_CC_E = 0,         /* correlation energy E1 + Eaa + Eab + Ebb */
+    DTCF_CC_E1 = 1,        /* sum fa_ov.ta + sum fb_ov.tb */
+    DTCF_CC_EAA = 2,       /* 1/4 sum vaa_oovv (taa + 2 ta ta) */
+    DTCF_CC_EAB = 3,       /* sum vab_oovv (tab + ta tb) */
+    DTCF_CC_EBB = 4,       /* 1/4 sum vbb_oovv (tbb + 2 tb tb) */
+    DTCF_CC_R1 = 5,        /* |R1| over both spins */
+    DTCF_CC_R2 = 6,        /* |R2| over the three blocks */
+    DTCF_CC_DT = 7,        /* |T_new - T| over every amplitude */
+    DTCF_CC_NOUT = 8
+};
+
+/* update != 0: t += r / D in place (D_ai = e_i - e_a, D_abij =
+ * ((e_i + e_j) - e_a) - e_b with each index's spin), then the energy of the
+ * updated amplitudes; update == 0: the energy of the amplitudes as they are
+ * (residuals not read, norms 0).  Writes DTCF_CC_NOUT doubles to out_dev
+ * (device).  Deterministic: the sums do not depend on timing. */
+int dtcf_uhf_ccsd_step_f64(const dtcf_uhf_ccsd *d, int update, double *out_dev, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* 8. Serving: device-resident CC iteration programs under an HBM      */
 /*    budget, continuous batching over streams.  Replaces DLTC's task  */
 /*    pool and tile cache (DLTC/task.cc:24-137 task_mul/task_sum and   */
@@ -450,15 +489,19 @@ int dtcf_program_create(dtcf_server s, int prec, int nslice, int max_level, dtcf
 /* Before the first statement: the m n / (m + n) from which a contraction
  * takes the program's mixed precision (0: every contraction). */
 int dtcf_program_set_mixed_min(dtcf_program p, double h);
-/* Before the first statement, mixed programs: 1 makes finalize split every
- * input operand of a mixed contraction once, from the bound input's contents
- * at that time, into the server's arena (counted as resident until the
- * program is destroyed, one slice set per contraction and operand view);
- * requests then split only the other operands.  The inputs must be written
- * before finalize and not change afterwards. */
-int dtcf_program_set_resident_slices(dtcf_program p, int on);
 int dtcf_program_array(dtcf_program p, const char *name, int ndim, const int64_t *dims, int role);
-/* Inputs are resident tensors the caller owns (e.g. from dtcf_server_alloc). */
+/* Mixed programs, after declaring input `name` and before the first
+ * statement: on != 0 makes finalize split that input, wherever a mixed
+ * contraction reads it, once (from its bound contents at that time) into the
+ * server's arena, counted as resident until the program is destroyed (one
+ * slice set per contraction and operand view); requests then split only the
+ * other operands.  For inputs that do not change between requests (the
+ * integrals of a CC iteration, not its amplitudes); such an input cannot be
+ * rebound after finalize. */
+int dtcf_program_set_resident(dtcf_program p, const char *name, int on);
+/* Inputs are resident tensors the caller owns (e.g. from dtcf_server_alloc).
+ * After finalize an input may be rebound while no request of the program is
+ * in flight, unless it has resident slices. */
 int dtcf_program_bind(dtcf_program p, const char *name, const double *dev_ptr);
 /* C[c_idx] = alpha * sum_{contracted} A[a_idx] * B[b_idx] */
 int dtcf_program_contract(dtcf_program p, const char *c, const char *c_idx, const char *a,

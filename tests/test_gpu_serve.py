@@ -120,16 +120,20 @@ def test_mixed_min_policy(dev, lists):
 
 
 def test_resident_slices_bitwise(dev, lists):
-    """Input operands split once at finalize give bitwise the results of
+    """Integral inputs split once at finalize give bitwise the results of
     splitting them in every request (same kernels, same digits), twice in a
-    row; the slices count as resident and are released with the program."""
+    row; the slices count as resident and are released with the program.
+    The amplitudes are never resident: they change every iteration."""
     tl = lists["t2"]
     sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 10}
     with S.Server(1 << 30, nslots=2) as srv:
         sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
         base = srv.usage()["resident"]
+        res = S.integral_inputs(tl)
+        assert res and not any(n.startswith("t") for n in res)
+        assert len(res) < len(tl.inputs())
         p0 = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0)
-        p1 = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0, resident_slices=True)
+        p1 = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0, resident=res)
         assert srv.usage()["resident"] > base
         r0 = p0.run()
         for _ in range(2):
@@ -139,6 +143,40 @@ def test_resident_slices_bitwise(dev, lists):
         p0.destroy()
         p1.destroy()
         assert srv.usage()["resident"] == base
+
+
+def test_rebind_after_finalize(dev, lists):
+    """An input with resident slices cannot be rebound after finalize (its
+    slices were cut from the old contents); another input can, between
+    requests, and the program then reads the new array: its results equal a
+    program finalized on the new array."""
+    import ctypes as C
+
+    tl = lists["t2"]
+    sizes = {"Oa": 4, "Ob": 4, "Va": 12, "Vb": 10}
+    with S.Server(1 << 30, nslots=2) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes, seed=5)
+        other = S.System(srv, 0, 0, [tl], sizes=sizes, seed=6)
+        res = S.integral_inputs(tl)
+        p = S.Program(srv, tl, sysm, prec=mixed.I8, nslice=5, mixed_min=0, resident=res)
+        p.run()
+        L = lib()
+        with pytest.raises(RuntimeError, match="resident slices"):
+            from dtcf_amd._lib import check
+            check(L.dtcf_program_bind(p.h, res[0].encode(), C.c_void_p(other.ptr[res[0]])))
+        amp = "taa_vvoo"
+        assert amp in tl.inputs() and amp not in res
+        check(L.dtcf_program_bind(p.h, amp.encode(), C.c_void_p(other.ptr[amp])))
+        got = p.run()
+        # reference: the same program over sysm's inputs with other's taa_vvoo
+        sysm2 = S.System(srv, 0, 0, [tl], sizes=sizes, seed=5)
+        S.from_host(sysm2.ptr[amp], other.host(amp))
+        pr = S.Program(srv, tl, sysm2, prec=mixed.I8, nslice=5, mixed_min=0, resident=res)
+        ref = pr.run()
+        for name in got:
+            assert np.array_equal(got[name], ref[name]), name
+        p.destroy()
+        pr.destroy()
 
 
 def _outputs_host(outs):

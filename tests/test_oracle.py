@@ -258,6 +258,82 @@ def test_ccd_iteration_vs_loops():
     assert abs(E - Eref) <= 1e-13 * max(1.0, abs(Eref))
 
 
+def _uhf_random(Oa, Ob, Va, Vb, seed, residuals=True):
+    rng = np.random.default_rng(seed)
+    a = {"eoa": -2.0 + 1.5 * rng.random(Oa), "eob": -2.0 + 1.5 * rng.random(Ob),
+         "eva": 0.2 + 2.8 * rng.random(Va), "evb": 0.2 + 2.8 * rng.random(Vb),
+         "fa_ov": rng.standard_normal((Oa, Va)), "fb_ov": rng.standard_normal((Ob, Vb)),
+         "vaa_oovv": rng.standard_normal((Oa, Oa, Va, Va)),
+         "vab_oovv": rng.standard_normal((Oa, Ob, Va, Vb)),
+         "vbb_oovv": rng.standard_normal((Ob, Ob, Vb, Vb))}
+    shapes = {"ta_vo": (Va, Oa), "tb_vo": (Vb, Ob), "taa_vvoo": (Va, Va, Oa, Oa),
+              "tab_vvoo": (Va, Vb, Oa, Ob), "tbb_vvoo": (Vb, Vb, Ob, Ob)}
+    for n, shp in shapes.items():
+        a[n] = 0.1 * rng.standard_normal(shp)
+        if residuals:
+            a["r" + n[1:]] = rng.standard_normal(shp)
+    return a
+
+
+def test_uhf_energy_vs_spin_orbital_embedding():
+    """The UHF block energy (factors 1/4 same spin, 1 mixed, the singles
+    products) equals the spin-orbital CCSD energy of the embedded tensors,
+    for antisymmetric and for unsymmetric same-spin blocks, before and after
+    an update."""
+    for anti in (False, True):
+        a = _uhf_random(3, 2, 4, 5, 21 + anti)
+        if anti:
+            for k in ("vaa_oovv", "vbb_oovv", "taa_vvoo", "tbb_vvoo"):
+                x = a[k]
+                a[k] = 0.5 * (x - x.transpose(1, 0, 2, 3) - x.transpose(0, 1, 3, 2)
+                              + x.transpose(1, 0, 3, 2))
+        for update in (False, True):
+            t, out = oracle.uhf_ccsd_step(a, update)
+            e_so = oracle.spin_orbital_energy(a, t)
+            assert abs(out[0] - e_so) <= 1e-13 * max(1.0, abs(e_so)), (anti, update)
+            assert abs(out[0] - out[1:5].sum()) <= 1e-14 * max(1.0, abs(out[0]))
+
+
+def test_uhf_update_loops_and_zero_start():
+    a = _uhf_random(2, 3, 4, 3, 5)
+    t, out = oracle.uhf_ccsd_step(a, True)
+    for n, eo, ev in (("ta_vo", a["eoa"], a["eva"]), ("tb_vo", a["eob"], a["evb"])):
+        r = a["r" + n[1:]]
+        for x, i in itertools.product(range(len(ev)), range(len(eo))):
+            assert t[n][x, i] == a[n][x, i] + r[x, i] / (eo[i] - ev[x])
+    for n, e1, e2, v1, v2 in (("taa_vvoo", "eoa", "eoa", "eva", "eva"),
+                              ("tab_vvoo", "eoa", "eob", "eva", "evb"),
+                              ("tbb_vvoo", "eob", "eob", "evb", "evb")):
+        r = a["r" + n[1:]]
+        for x, y, i, j in itertools.product(*[range(k) for k in r.shape]):
+            d = ((a[e1][i] + a[e2][j]) - a[v1][x]) - a[v2][y]
+            assert t[n][x, y, i, j] == a[n][x, y, i, j] + r[x, y, i, j] / d
+    r1 = np.sqrt(sum(np.sum(a[k] ** 2) for k in ("ra_vo", "rb_vo")))
+    r2 = np.sqrt(sum(np.sum(a[k] ** 2) for k in ("raa_vvoo", "rab_vvoo", "rbb_vvoo")))
+    dt = np.sqrt(sum(np.sum((t[k] - a[k]) ** 2) for k in t))
+    assert abs(out[5] - r1) <= 1e-13 * r1 and abs(out[6] - r2) <= 1e-13 * r2
+    assert abs(out[7] - dt) <= 1e-12 * dt
+    # zero amplitudes, residuals f_vo and v_vvoo: first-order amplitudes, and
+    # the energy is sum f_ia f_ai / D_ai + 1/4 sum |v|^2 / D (same spin) ...
+    for n in ("ta_vo", "tb_vo", "taa_vvoo", "tab_vvoo", "tbb_vvoo"):
+        a[n] = np.zeros_like(a[n])
+    a["ra_vo"], a["rb_vo"] = a["fa_ov"].T.copy(), a["fb_ov"].T.copy()
+    a["raa_vvoo"] = a["vaa_oovv"].transpose(2, 3, 0, 1).copy()
+    a["rab_vvoo"] = a["vab_oovv"].transpose(2, 3, 0, 1).copy()
+    a["rbb_vvoo"] = a["vbb_oovv"].transpose(2, 3, 0, 1).copy()
+    t, out = oracle.uhf_ccsd_step(a, True)
+    Da = a["eoa"][None, :] - a["eva"][:, None]
+    Db = a["eob"][None, :] - a["evb"][:, None]
+    e1 = np.sum(a["fa_ov"].T ** 2 / Da) + np.sum(a["fb_ov"].T ** 2 / Db)
+    assert abs(out[1] - e1) <= 1e-13 * abs(e1)
+    ta, tb = a["fa_ov"].T / Da, a["fb_ov"].T / Db
+    Dab = (a["eoa"][None, None, :, None] + a["eob"][None, None, None, :]
+           - a["eva"][:, None, None, None] - a["evb"][None, :, None, None])
+    eab = np.sum(a["vab_oovv"].transpose(2, 3, 0, 1) ** 2 / Dab) + \
+        np.einsum("ijab,ai,bj->", a["vab_oovv"], ta, tb)
+    assert abs(out[3] - eab) <= 1e-12 * abs(eab)
+
+
 def test_permutation_maps_c1_c2():
     # C2: contract("i,k,a,c", "k,j,c,b", "i,j,a,b")
     br = dict(i=12, j=12, k=12, a=48, b=48, c=48)
