@@ -113,7 +113,6 @@ class CCSystem(serve.System):
             block = name.split("_")[1]
             if block in ("oo", "vv"):
                 eps = self.eps[("eo" if block == "oo" else "ev") + spin]
-                n = shp[0]
                 N = self._normal(name, shp, 1.0)
                 N = (N + N.t()) / math.sqrt(2.0)
                 N.fill_diagonal_(0.0)
@@ -212,3 +211,114 @@ class UHFCCSD:
         for p in self.progs:
             p.destroy()
         self.progs = []
+
+
+MODES = {
+    "fp64": (0, 0, 0),
+    "int8x5": (mixed.I8, 5, 0),
+    "int8x4/L6": (mixed.I8, 4, 6),
+}
+
+
+def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb=240.0, nslots=8,
+          seed=11, log=print):
+    """Full UHF CCSD iterations, FP64 against mixed residual programs.
+
+    Starting from the first-order amplitudes (one FP64 iteration from zero),
+    each of ``iters`` iterations runs every mixed mode from the FP64
+    trajectory's amplitudes (restored before each run), then the FP64
+    iteration itself, so every mode's error is that of one iteration from the
+    same state.  Per mode: seconds per iteration (both residual programs, then
+    the update / energy step), contraction TFLOP/s (dense flops of the two
+    programs over the residual time), speedup over FP64, and the largest
+    |dE| (total and per component E1, Eaa, Eab, Ebb) and max |dR| / max |R|
+    per residual block over the iterations."""
+    srv = serve.Server(int(budget_gb * (1 << 30)), nslots)
+    try:
+        t = time.perf_counter()
+        system = CCSystem(srv, o, v, seed=seed)
+        torch.cuda.empty_cache()
+        log(f"system o={o} v={v}: inputs {system.bytes() / 1e9:.2f} GB in "
+            f"{time.perf_counter() - t:.1f} s")
+        drv = {}
+        for m in modes:
+            prec, ns, lev = MODES[m]
+            drv[m] = UHFCCSD(srv, system, prec=prec, nslice=ns, max_level=lev)
+            log(f"{m}: programs planned; resident {srv.usage()['resident'] / 1e9:.2f} GB")
+        ref = drv["fp64"]
+        first = ref.iterate()
+        log(f"first-order amplitudes: E {first['E']:.10f}")
+        rec = {m: {"seconds": [], "seconds_residuals": [], "E": [], "dE": [], "dR": []} for m in modes}
+        for it in range(iters):
+            start = {n: torch.empty(int(np.prod(system.shape[n])), dtype=torch.float64,
+                                    device="cuda") for n in AMPS}
+            for n, x in start.items():
+                serve.copy_device(x.data_ptr(), system.ptr[n], x.numel())
+            results, resid = {}, {}
+            for m in [x for x in modes if x != "fp64"] + ["fp64"]:
+                for n, x in start.items():
+                    serve.copy_device(system.ptr[n], x.data_ptr(), x.numel())
+                torch.cuda.synchronize()
+                r = drv[m].iterate()
+                results[m] = r
+                resid[m] = drv[m].res
+                rec[m]["seconds"].append(r["seconds"])
+                rec[m]["seconds_residuals"].append(r["seconds_residuals"])
+                rec[m]["E"].append(r["E"])
+            e64 = results["fp64"]
+            for m in modes:
+                dE = {k: abs(results[m][k] - e64[k]) for k in ("E", "E1", "Eaa", "Eab", "Ebb")}
+                dR = {n: ((resid[m][n] - resid["fp64"][n]).abs().max() /
+                          resid["fp64"][n].abs().max()).item() for n in resid[m]}
+                rec[m]["dE"].append(dE)
+                rec[m]["dR"].append(dR)
+            log(f"iteration {it}: E(fp64) {e64['E']:.10f}, |R2| {e64['r2']:.3e}; " + ", ".join(
+                f"{m} {results[m]['seconds']:.3f} s dE {rec[m]['dE'][-1]['E']:.1e}" for m in modes))
+            del start
+        out = {"o": o, "v": v, "iterations": iters, "flops_per_iteration": ref.flops,
+               "energy_first_order": first["E"], "modes": {}}
+        t64 = float(np.median(rec["fp64"]["seconds"]))
+        for m in modes:
+            r = rec[m]
+            s = float(np.median(r["seconds"]))
+            sr = float(np.median(r["seconds_residuals"]))
+            out["modes"][m] = {
+                "seconds_per_iteration": s, "seconds_residuals": sr,
+                "tflops_residuals": ref.flops / sr / 1e12, "speedup": t64 / s,
+                "energy": r["E"],
+                "max_energy_err": {k: max(d[k] for d in r["dE"]) for k in r["dE"][0]},
+                "max_residual_rel_err": {k: max(d[k] for d in r["dR"]) for k in r["dR"][0]},
+            }
+        out["usage_gb"] = {k: v / 1e9 for k, v in srv.usage().items()}
+        for d in drv.values():
+            d.destroy()
+        return out
+    finally:
+        srv.close()
+
+
+def main():
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--o", type=int, default=64)
+    ap.add_argument("--v", type=int, default=256)
+    ap.add_argument("--iters", type=int, default=3)
+    ap.add_argument("--budget-gb", type=float, default=240.0)
+    ap.add_argument("--modes", default="fp64,int8x5,int8x4/L6")
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    check(lib().dtcf_init(0))
+    res = study(args.o, args.v, args.iters, tuple(args.modes.split(",")), args.budget_gb,
+                log=lambda s: print(s, flush=True))
+    print(json.dumps(res), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(res, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

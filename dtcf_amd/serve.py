@@ -97,6 +97,14 @@ def from_host(ptr, arr):
             raise RuntimeError(f"hipMemcpy host->device failed ({st})")
 
 
+def copy_device(dst, src, n):
+    """n doubles device -> device (raw addresses)."""
+    if n:
+        st = _hip().hipMemcpy(C.c_void_p(dst), C.c_void_p(src), n * 8, 3)
+        if st != 0:
+            raise RuntimeError(f"hipMemcpy device->device failed ({st})")
+
+
 def system_sizes(o, v):
     assert o % 2 == 0 and v % 2 == 0, "spin-orbital counts split evenly into alpha and beta"
     return {"Oa": o // 2, "Ob": o // 2, "Va": v // 2, "Vb": v // 2}
