@@ -24,6 +24,8 @@ int hip_fail(hipError_t e, const char *what)
 
 static std::atomic<int> g_cfg{-1};
 
+int ccsd_scratch_setup(int dev);  // ccsd.hip
+
 // Tile-shape choice when no explicit configuration is set.  The table is
 // filled by the autotuner (dtcf_amd/autotune.py) from measurements on the
 // device; the defaults keep 128x128 everywhere except thin problems, where a
@@ -54,7 +56,7 @@ int dtcf_init(int device)
 {
     DTCF_HIP(hipSetDevice(device));
     DTCF_HIP(hipFree(nullptr));
-    return DTCF_OK;
+    return ccsd_scratch_setup(device);
 }
 
 int dtcf_gemm_set_config(int cfg)

@@ -344,10 +344,59 @@ int stream_partials(hipStream_t s, double **p)
     return DTCF_OK;
 }
 
+// Partial sums of the allocation-free entry point: one array per device for
+// the life of the process (PART_DOUBLES doubles = 147 KiB, whatever the
+// system size).  Host code never passes `scratch` itself to a kernel: its
+// device address comes from hipGetSymbolAddress, once per device.
+__device__ __align__(256) double scratch[PART_DOUBLES];
+
+constexpr int kMaxDev = 64;
+std::mutex g_scr_mu;
+double *g_scr[kMaxDev] = {};
+
+int scratch_for_device(int dev, double **p)
+{
+    DTCF_REQUIRE(dev >= 0 && dev < kMaxDev, "uhf_ccsd: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (!g_scr[dev]) {
+        void *ptr = nullptr;
+        DTCF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        DTCF_REQUIRE(ptr, "uhf_ccsd: no device address for the scratch array");
+        g_scr[dev] = static_cast<double *>(ptr);
+    }
+    *p = g_scr[dev];
+    return DTCF_OK;
+}
+
 }  // namespace
+
+// Called by dtcf_init with `dev` current: resolves that device's scratch
+// address up front, so the allocation-free entry point only looks it up.
+int ccsd_scratch_setup(int dev)
+{
+    double *p = nullptr;
+    return scratch_for_device(dev, &p);
+}
+
 }  // namespace dtcf
 
 using namespace dtcf;
+
+extern "C" int dtcf_uhf_ccsd_step_f64_noalloc(const dtcf_uhf_ccsd *d, int update, double *out_dev,
+                                              void *stream)
+{
+    int st = validate(d, update);
+    if (st != DTCF_OK) return st;
+    DTCF_REQUIRE(out_dev, "uhf_ccsd: null output");
+    int dev = 0;
+    DTCF_HIP(hipGetDevice(&dev));
+    double *part = nullptr;
+    st = scratch_for_device(dev, &part);  // set up by dtcf_init, else on the first call here
+    if (st != DTCF_OK) return st;
+    // every partial the finish pass reads is written by this call's kernels,
+    // so nothing in the shared scratch has to be cleared first
+    return launch_step(d, update, part, out_dev, as_stream(stream));
+}
 
 extern "C" int dtcf_uhf_ccsd_step_f64(const dtcf_uhf_ccsd *d, int update, double *out_dev,
                                       void *stream)

@@ -450,6 +450,18 @@ enum {
  * (device).  Deterministic: the sums do not depend on timing. */
 int dtcf_uhf_ccsd_step_f64(const dtcf_uhf_ccsd *d, int update, double *out_dev, void *stream);
 
+/* The same step without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * partial sums live in a fixed device array of the library (147 KiB per
+ * device, held until the process ends, enough for any system size the
+ * ranges allow), resolved by dtcf_init or on the first call on a device.
+ * Every call on one device shares that array: calls that overlap on one
+ * device (other streams, other threads) race and must be serialised by the
+ * caller.  Results are bitwise those of dtcf_uhf_ccsd_step_f64. */
+int dtcf_uhf_ccsd_step_f64_noalloc(const dtcf_uhf_ccsd *d, int update, double *out_dev,
+                                   void *stream);
+
 /* ------------------------------------------------------------------ */
 /* 8. Serving: device-resident CC iteration programs under an HBM      */
 /*    budget, continuous batching over streams.  Replaces DLTC's task  */
