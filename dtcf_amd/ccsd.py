@@ -217,11 +217,13 @@ MODES = {
     "fp64": (0, 0, 0),
     "int8x5": (mixed.I8, 5, 0),
     "int8x4/L6": (mixed.I8, 4, 6),
+    "int8x4/L5": (mixed.I8, 4, 5),
+    "int8x3/L4": (mixed.I8, 3, 4),
 }
 
 
 def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb=240.0, nslots=8,
-          seed=11, log=print):
+          seed=11, sigma=0.05, log=print):
     """Full UHF CCSD iterations, FP64 against mixed residual programs.
 
     Starting from the first-order amplitudes (one FP64 iteration from zero),
@@ -236,7 +238,7 @@ def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb
     srv = serve.Server(int(budget_gb * (1 << 30)), nslots)
     try:
         t = time.perf_counter()
-        system = CCSystem(srv, o, v, seed=seed)
+        system = CCSystem(srv, o, v, seed=seed, sigma=sigma)
         torch.cuda.empty_cache()
         log(f"system o={o} v={v}: inputs {system.bytes() / 1e9:.2f} GB in "
             f"{time.perf_counter() - t:.1f} s")
@@ -275,7 +277,7 @@ def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb
             log(f"iteration {it}: E(fp64) {e64['E']:.10f}, |R2| {e64['r2']:.3e}; " + ", ".join(
                 f"{m} {results[m]['seconds']:.3f} s dE {rec[m]['dE'][-1]['E']:.1e}" for m in modes))
             del start
-        out = {"o": o, "v": v, "iterations": iters, "flops_per_iteration": ref.flops,
+        out = {"o": o, "v": v, "sigma": sigma, "iterations": iters, "flops_per_iteration": ref.flops,
                "energy_first_order": first["E"], "modes": {}}
         t64 = float(np.median(rec["fp64"]["seconds"]))
         for m in modes:
@@ -307,12 +309,13 @@ def main():
     ap.add_argument("--v", type=int, default=256)
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--budget-gb", type=float, default=240.0)
+    ap.add_argument("--sigma", type=float, default=0.05)
     ap.add_argument("--modes", default="fp64,int8x5,int8x4/L6")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     check(lib().dtcf_init(0))
     res = study(args.o, args.v, args.iters, tuple(args.modes.split(",")), args.budget_gb,
-                log=lambda s: print(s, flush=True))
+                sigma=args.sigma, log=lambda s: print(s, flush=True))
     print(json.dumps(res), flush=True)
     if args.out:
         with open(args.out, "w") as f:

@@ -28,8 +28,9 @@ int ccsd_scratch_setup(int dev);  // ccsd.hip
 
 // Tile-shape choice when no explicit configuration is set.  The table is
 // filled by the autotuner (dtcf_amd/autotune.py) from measurements on the
-// device; the defaults keep 128x128 everywhere except thin problems, where a
-// smaller tile gives more workgroups.
+// device; the defaults keep 128x128 everywhere except small grids, where a
+// smaller tile gives more workgroups, and thin results (64 or 96 wide), whose
+// tile matches their width.
 int pick_config(int64_t m, int64_t n, int64_t k)
 {
     const int forced = g_cfg.load();
@@ -37,6 +38,10 @@ int pick_config(int64_t m, int64_t n, int64_t k)
     (void)k;
     const int64_t tiles = ((m + 127) / 128) * ((n + 127) / 128);
     if (tiles < 256) return kCfg64x64;
+    // thin results: a 128-wide tile would be mostly padding
+    if (n <= 64 || m <= 64) return kCfg64x64;
+    if (n <= 96) return kCfg192x96;
+    if (m <= 96) return kCfg96x192;
     return kCfg128x128;
 }
 

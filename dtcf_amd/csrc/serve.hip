@@ -98,6 +98,90 @@ __global__ __launch_bounds__(256) void sum_kernel(SumArgs a)
     }
 }
 
+// The same sums, one (i2, i3) plane of the output per workgroup pass: the
+// result element (i0, i1, i2, i3) adds its terms in list order exactly as
+// sum_kernel does.  Terms go in chunks of kChunk: every term of a chunk is
+// loaded before one barrier, so kChunk terms' loads are in flight at once.
+// A term whose unit stride lies on i2 (the occupied pair transposed, e.g.
+// X[p2,p1,h2,h1] into R[p1,p2,h1,h2]) is staged through its own LDS plane
+// with lanes along i2; any other term is read directly into registers
+// (lanes along i3: whole lines when its unit stride lies there).  (i2, i3)
+// come from 32-bit divisions by invariants, (i0, i1) once per plane.
+constexpr int kPlaneCap = 3072;  // doubles per staged plane: d3 rows of d2 + 1
+constexpr int kPlanePer = 12;    // result elements per thread: d2 * d3 <= 3072
+constexpr int kChunk = 4;        // terms per barrier
+
+__global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d3, FastDiv div_d2,
+                                                        uint64_t nplanes, uint32_t bufstride)
+{
+    extern __shared__ double planes[];  // kChunk staged planes of bufstride doubles
+    __shared__ int64_t st[kMaxSumTerms * 4];
+    __shared__ const double *src[kMaxSumTerms];
+    __shared__ double coef[kMaxSumTerms];
+    for (int i = threadIdx.x; i < a.nterms * 4; i += 256) st[i] = a.strides[i];
+    if (threadIdx.x < a.nterms) {
+        src[threadIdx.x] = a.src[threadIdx.x];
+        coef[threadIdx.x] = a.coef[threadIdx.x];
+    }
+    __syncthreads();
+    const uint32_t d2 = (uint32_t)a.d2, d3 = (uint32_t)a.d3, P = d2 * d3, pitch = d2 + 1;
+    uint32_t i2[kPlanePer], i3[kPlanePer];
+#pragma unroll
+    for (int k = 0; k < kPlanePer; k++) {
+        const uint32_t e = min<uint32_t>(threadIdx.x + 256u * k, P - 1);
+        i2[k] = div_d3.div(e);
+        i3[k] = e - i2[k] * d3;
+    }
+    for (uint64_t plane = blockIdx.x; plane < nplanes; plane += gridDim.x) {
+        const uint64_t i0 = plane / (uint64_t)a.d1, i1 = plane - i0 * (uint64_t)a.d1;
+        double acc[kPlanePer];
+#pragma unroll
+        for (int k = 0; k < kPlanePer; k++) acc[k] = 0.0;
+        for (int t0 = 0; t0 < a.nterms; t0 += kChunk) {
+            double xr[kChunk][kPlanePer];
+            bool staged[kChunk];
+            __syncthreads();  // the previous chunk's staged planes have been read
+#pragma unroll
+            for (int j = 0; j < kChunk; j++) {
+                const int t = t0 + j;
+                staged[j] = false;
+                if (t >= a.nterms) continue;
+                const int64_t *q = st + 4 * t;
+                const double *x = src[t] + (int64_t)i0 * q[0] + (int64_t)i1 * q[1];
+                if (q[2] == 1 && q[3] != 1) {
+                    staged[j] = true;
+                    double *b = planes + j * bufstride;
+                    for (uint32_t idx = threadIdx.x; idx < P; idx += 256) {
+                        const uint32_t r3 = div_d2.div(idx), r2 = idx - r3 * d2;
+                        b[r3 * pitch + r2] = x[r2 + (int64_t)r3 * q[3]];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kPlanePer; k++)
+                        xr[j][k] = threadIdx.x + 256u * k < P
+                                       ? x[(int64_t)i2[k] * q[2] + (int64_t)i3[k] * q[3]]
+                                       : 0.0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kChunk; j++) {
+                const int t = t0 + j;
+                if (t >= a.nterms) continue;
+                const double c = coef[t];
+                const double *b = planes + j * bufstride;
+#pragma unroll
+                for (int k = 0; k < kPlanePer; k++)
+                    acc[k] += c * (staged[j] ? b[i3[k] * pitch + i2[k]] : xr[j][k]);
+            }
+        }
+        double *o = a.out + plane * P;
+#pragma unroll
+        for (int k = 0; k < kPlanePer; k++)
+            if (threadIdx.x + 256u * k < P) o[threadIdx.x + 256u * k] = acc[k];
+    }
+}
+
 unsigned grid_for(int64_t n)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
@@ -217,6 +301,12 @@ struct PStmt {
     bool res[2] = {false, false};
     char *rs[2] = {nullptr, nullptr};
     int32_t *re[2] = {nullptr, nullptr};
+    // mixed, input operand q: its view as (rows, K, row offsets, K offsets);
+    // operands with equal keys on one array have equal slices
+    std::vector<int64_t> vkey[2];
+    // slices of operand q made by an earlier statement of the request:
+    // share_st[q] (its operand share_q[q]), or -1
+    int share_st[2] = {-1, -1}, share_q[2] = {0, 0};
     // the contraction runs in the program's mixed precision (else FP64)
     bool mixed = false;
     // FP64: operands in GEMM position 0 (A) / 1 (B) packed into scratch first
@@ -228,6 +318,7 @@ struct PStmt {
     std::vector<double> coefs;
     int64_t od[4] = {1, 1, 1, 1};
     int64_t *dstrides = nullptr;  // device [nterms][4]
+    bool plane = false;           // sum_plane_kernel: (od[2] + 1) * od[3] <= kPlaneCap
     std::vector<int> deps;
     std::vector<int> succ;
     double flops = 0;
@@ -476,8 +567,18 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
         DTCF_REQUIRE(s.sa_bytes > 0 && s.sb_bytes > 0, "program: bad mixed-precision mode");
         // GEMM operand 0 is array b when swapped, a otherwise
         const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
-        for (int q = 0; q < 2; q++)
+        for (int q = 0; q < 2; q++) {
             s.res[q] = p->arrs[arr[q]].role == DTCF_ARR_INPUT && p->arrs[arr[q]].resident;
+            if (p->arrs[arr[q]].role != DTCF_ARR_INPUT) continue;
+            const auto &R = q == 0 ? Ar : Br;
+            const auto &Kk = q == 0 ? Ak : Bk;
+            auto &key = s.vkey[q];
+            key.reserve(2 + R.size() + Kk.size());
+            key.push_back((int64_t)R.size());
+            key.push_back((int64_t)Kk.size());
+            key.insert(key.end(), R.begin(), R.end());
+            key.insert(key.end(), Kk.begin(), Kk.end());
+        }
         int64_t off = 0;
         if (!s.res[0]) { s.a_off = off; off += s.sa_bytes; }
         if (!s.res[1]) { s.b_off = off; off += s.sb_bytes; }
@@ -515,7 +616,25 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
             a.src[t] = arr_ptr(p, p->arrs[s.terms[t]], r);
             a.coef[t] = s.coefs[t];
         }
-        hipLaunchKernelGGL(sum_kernel, dim3(grid_for(a.n)), dim3(256), 0, stream, a);
+        if (s.plane) {
+            static std::once_flag once;
+            static hipError_t attr = hipSuccess;
+            std::call_once(once, [] {
+                attr = hipFuncSetAttribute(reinterpret_cast<const void *>(sum_plane_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(kChunk * kPlaneCap * sizeof(double)));
+            });
+            DTCF_HIP(attr);
+            const uint64_t nplanes = (uint64_t)(a.n / (s.od[2] * s.od[3]));
+            const unsigned grid = (unsigned)std::min<uint64_t>(nplanes, 1u << 20);
+            const uint32_t bufstride = (uint32_t)(s.od[3] * (s.od[2] + 1));
+            hipLaunchKernelGGL(sum_plane_kernel, dim3(grid), dim3(256),
+                               (size_t)kChunk * bufstride * sizeof(double), stream, a,
+                               FastDiv((uint32_t)s.od[3]), FastDiv((uint32_t)s.od[2]), nplanes,
+                               bufstride);
+        } else {
+            hipLaunchKernelGGL(sum_kernel, dim3(grid_for(a.n)), dim3(256), 0, stream, a);
+        }
         DTCF_HIP(hipGetLastError());
         return DTCF_OK;
     }
@@ -538,13 +657,29 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
     // mixed precision accumulates its products into a cleared C
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(C.elems)), dim3(256), 0, stream, c, C.elems);
     DTCF_HIP(hipGetLastError());
-    char *scr = r->base + s.scratch_off;
-    char *sa = s.res[0] ? s.rs[0] : scr + s.a_off, *sb = s.res[1] ? s.rs[1] : scr + s.b_off;
-    int32_t *ea = s.res[0] ? s.re[0] : reinterpret_cast<int32_t *>(scr + s.ea_off);
-    int32_t *eb = s.res[1] ? s.re[1] : reinterpret_cast<int32_t *>(scr + s.eb_off);
+    char *sl[2];
+    int32_t *ex[2];
+    bool split[2];
+    for (int q = 0; q < 2; q++) {
+        if (s.res[q]) {
+            sl[q] = s.rs[q];
+            ex[q] = s.re[q];
+            split[q] = false;
+            continue;
+        }
+        // own scratch, or the scratch of the statement whose slices it shares
+        const PStmt &o = s.share_st[q] >= 0 ? p->st[s.share_st[q]] : s;
+        const int oq = s.share_st[q] >= 0 ? s.share_q[q] : q;
+        char *scr = r->base + o.scratch_off;
+        sl[q] = scr + (oq == 0 ? o.a_off : o.b_off);
+        ex[q] = reinterpret_cast<int32_t *>(scr + (oq == 0 ? o.ea_off : o.eb_off));
+        split[q] = s.share_st[q] < 0;
+    }
+    char *sa = sl[0], *sb = sl[1];
+    int32_t *ea = ex[0], *eb = ex[1];
     int st = DTCF_OK;
-    if (!s.res[0]) st = dtcf_split_f64(p->prec, p->nslice, A, s.av, sa, ea, stream);
-    if (st == DTCF_OK && !s.res[1]) st = dtcf_split_f64(p->prec, p->nslice, B, s.bv, sb, eb, stream);
+    if (split[0]) st = dtcf_split_f64(p->prec, p->nslice, A, s.av, sa, ea, stream);
+    if (st == DTCF_OK && split[1]) st = dtcf_split_f64(p->prec, p->nslice, B, s.bv, sb, eb, stream);
     if (st == DTCF_OK)
         st = dtcf_gemm_split_f64(p->prec, p->nslice, s.m, s.n, s.k, s.alpha, sa, ea, sb, eb,
                                  p->max_level, c, s.cv, s.ldc, stream);
@@ -974,6 +1109,7 @@ extern "C" int dtcf_program_sum(dtcf_program p, const char *c, const char *c_idx
     DTCF_HIP(hipMalloc(&s.dstrides, strides.size() * sizeof(int64_t)));
     DTCF_HIP(hipMemcpy(s.dstrides, strides.data(), strides.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     s.flops = (double)C.elems * nterms;
+    s.plane = (s.od[2] + 1) * s.od[3] <= kPlaneCap && s.od[2] * s.od[3] <= 256 * kPlanePer;
     p->st.push_back(std::move(s));
     p->broken = false;
     return DTCF_OK;
@@ -1008,6 +1144,34 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
     // The separate layout is taken when its block is within 1.5x (or
     // 256 MiB) of the reuse layout's: small programs then run as wide as
     // their data dependencies allow, large ones keep the smaller block.
+    // Input operands split per request: a later statement reading the same
+    // array through the same view takes the first one's slices (after it)
+    // instead of splitting again; the first one's scratch then stays placed
+    // until its last sharer.
+    std::vector<std::vector<int>> sharers(ns);
+    {
+        std::map<std::pair<int, std::vector<int64_t>>, std::pair<int, int>> first;
+        for (int i = 0; i < ns; i++) {
+            PStmt &s = p->st[i];
+            if (s.kind != 0 || !s.mixed) continue;
+            const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
+            std::pair<int, int> mine[2] = {{-1, 0}, {-1, 0}};
+            for (int q = 0; q < 2; q++) {
+                if (s.res[q] || s.vkey[q].empty()) continue;
+                auto it = first.find({arr[q], s.vkey[q]});
+                if (it != first.end()) {
+                    s.share_st[q] = it->second.first;
+                    s.share_q[q] = it->second.second;
+                    s.deps.push_back(it->second.first);
+                    sharers[it->second.first].push_back(i);
+                } else {
+                    mine[q] = {i, q};
+                }
+            }
+            for (int q = 0; q < 2; q++)
+                if (mine[q].first >= 0) first.emplace(std::make_pair(arr[q], s.vkey[q]), mine[q]);
+        }
+    }
     std::vector<std::vector<int>> dies(ns);
     for (int a = 0; a < (int)p->arrs.size(); a++) {
         const PArr &x = p->arrs[a];
@@ -1033,6 +1197,7 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
                         if (w != i) war[i].push_back(w);
             return off;
         };
+        std::vector<std::vector<int>> held(ns);  // scratch released after statement i
         for (int i = 0; i < ns; i++) {
             PStmt &s = p->st[i];
             PArr &C = p->arrs[s.c];
@@ -1040,11 +1205,22 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
             if (s.scratch_bytes > 0) {
                 s.scratch_off = place(i, s.scratch_bytes);
                 if (reuse) {
-                    plan.release(s.scratch_off, s.scratch_bytes);
-                    dead.push_back({s.scratch_off, granule(s.scratch_bytes), {i}});
+                    if (sharers[i].empty()) {
+                        plan.release(s.scratch_off, s.scratch_bytes);
+                        dead.push_back({s.scratch_off, granule(s.scratch_bytes), {i}});
+                    } else {
+                        held[sharers[i].back()].push_back(i);
+                    }
                 }
             }
             if (!reuse) continue;
+            for (int j : held[i]) {
+                const PStmt &o = p->st[j];
+                plan.release(o.scratch_off, o.scratch_bytes);
+                std::vector<int> waits{j};
+                waits.insert(waits.end(), sharers[j].begin(), sharers[j].end());
+                dead.push_back({o.scratch_off, granule(o.scratch_bytes), waits});
+            }
             for (int a : dies[i]) {
                 PArr &x = p->arrs[a];
                 plan.release(x.off, x.elems * (int64_t)sizeof(double));
@@ -1081,13 +1257,20 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
         }
         p->flops += s.flops;
     }
-    // Resident slices: each mixed contraction's input operands split once,
-    // from the bound inputs as they are now, into the server's arena.
+    // Resident slices: each resident input, once per distinct view, split
+    // from its bound contents as they are now into the server's arena.
+    std::map<std::pair<int, std::vector<int64_t>>, std::pair<char *, int32_t *>> made;
     for (auto &s : p->st) {
         if (s.kind != 0 || !s.mixed) continue;
         const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
         for (int q = 0; q < 2; q++) {
             if (!s.res[q]) continue;
+            auto hit = made.find({arr[q], s.vkey[q]});
+            if (hit != made.end()) {
+                s.rs[q] = hit->second.first;
+                s.re[q] = hit->second.second;
+                continue;
+            }
             const int64_t rows = q == 0 ? s.m : s.n;
             const int64_t sbytes = q == 0 ? s.sa_bytes : s.sb_bytes;
             void *ptr = nullptr;
@@ -1105,6 +1288,7 @@ extern "C" int dtcf_program_finalize(dtcf_program p)
                 p->broken = true;
                 return st;
             }
+            made.emplace(std::make_pair(arr[q], s.vkey[q]), std::make_pair(s.rs[q], s.re[q]));
         }
     }
     if (!p->res_allocs.empty()) DTCF_HIP(hipStreamSynchronize(nullptr));
