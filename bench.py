@@ -9,8 +9,10 @@ one dtcf_step_run: the transposes into GEMM form, the GEMM and the C
 update, with every input already resident in HBM.  The metric is FP64
 contraction TFLOP/s = 2*(o*v)^3 / step time.
 
-Also measured (extra keys of the same line): the precision study (FP64 vs
-FP32 / INT8 digit splits, ladder + ring at o=64, v=256), the block-sparse D2h
+Also measured (extra keys of the same line): full UHF CCSD iterations, FP64
+against mixed-precision residual programs (time per iteration, energy error
+per component), the precision study (FP64 vs FP32 / BF16 / FP16 / INT8 digit
+splits, ladder + ring at o=64, v=256, errors per term), the block-sparse D2h
 terms as grouped launches, and the served UHF CCSD stream (throughput and
 p50 / p99 latency).  --no-extra skips them.
 
@@ -32,9 +34,10 @@ sys.path.insert(0, ROOT)
 METRIC = "contraction TFLOP/s FP64 vs mixed at stated error; task p50/p99 latency"
 FP64_PEAK_TF = 78.6      # MI355X dense FP64 MFMA
 HBM_PEAK_GBS = 8000.0
-# measured HBM bytes per GEMM launch, keyed (mode, o, v): rocprofv3 FETCH_SIZE x2 +
-# WRITE_SIZE x1 (factors from a plain 2 GiB copy in the same run)
-TRAFFIC_BYTES = {("fused", 48, 192): 36.04e9}
+# HBM bytes per GEMM launch come from rocprofv3 --pmc runs of this script
+# (FETCH_SIZE, then WRITE_SIZE, scaled by a plain copy's counts), recorded in
+# profiles/traffic.json with the kernel's average duration in those runs.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def sig(x, n=4):
@@ -60,14 +63,53 @@ def make_inputs(o, v, vg):
     return out, addrs
 
 
+def host_cores():
+    """Cores this process may use, counted at run time: the physical cores
+    (distinct (package, core) pairs of /proc/cpuinfo) among the CPUs of its
+    affinity mask, capped by the cgroup CPU quota when one is set.  Returns
+    (count, description)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    phys, cur = {}, {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if "processor" in cur:
+                        phys[cur["processor"]] = (cur.get("physical id", "0"), cur.get("core id", cur["processor"]))
+                    cur = {}
+                    continue
+                k, val = (x.strip() for x in line.split(":", 1))
+                cur[k] = val
+        if "processor" in cur:
+            phys[cur["processor"]] = (cur.get("physical id", "0"), cur.get("core id", cur["processor"]))
+    except OSError:
+        pass
+    cores = {phys.get(str(c), ("0", str(c))) for c in allowed}
+    n = max(1, len(cores))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    desc = (f"{len(allowed)} logical CPUs in the affinity mask, {len(cores)} physical cores"
+            + (f", cgroup quota {quota} CPUs" if quota else ""))
+    return (min(n, quota) if quota else n), desc
+
+
 def cpu_baseline(A_t, B_t, C_t, o, v, vg, target_s=12.0):
-    """The restated reference CPU path (oracle, OpenMP) on this host:
-    tile transposes (helper.cc:275-522) + create_big_matrix packs
-    (transpose_helper.cc:20-171) in full, and the big dgemm on a row slab of
-    bigA sized to ~target_s, extrapolated to the full M."""
+    """The restated reference CPU path (oracle, OpenMP) on this host, the
+    reference's tr_time / comp_time split (contraction.cc:2923-2926) kept:
+    tile transposes (helper.cc:275-522) and create_big_matrix packs
+    (transpose_helper.cc:20-171) of A, B and C in full, the big dgemm on a
+    row slab of bigA sized to ~target_s and extrapolated to the full M, then
+    revert_big_matrix of C (transpose_helper.cc:331-409) and the inverse
+    transposes of its tiles (contraction.cc:2817-2830) in full."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads, core_desc = host_cores()
     br = [o // vg[0], o // vg[1], v // vg[2], v // vg[3]]
     bs = int(np.prod(br))
     ntiles = len(A_t) // bs
@@ -113,19 +155,102 @@ def cpu_baseline(A_t, B_t, C_t, o, v, vg, target_s=12.0):
                             bigB.ravel(), K, cs, N, threads)
     t_gemm_slab = time.perf_counter() - t0
     t_gemm = t_gemm_slab * M / rows
-    # unpack + inverse transposes of C (same traffic as the pack/transpose of C)
-    t_total = t_tr + t_pack + t_gemm + (t_tr + t_pack) / 3.0
+    del bigA, bigB
+    # revert_big_matrix of C, then the inverse transposes of its tiles
+    t0 = time.perf_counter()
+    tiles_c = oracle.big_matrix_unpack(bigC, oc, nrc, ncc, rpb, rpb, ntiles)
+    t_unpack = time.perf_counter() - t0
+    del bigC
+    inv = [0] * 4
+    for d in range(4):
+        inv[pm["p_map_C"][d]] = d
+    br_perm = [0] * 4
+    for d in range(4):
+        br_perm[pm["p_map_C"][d]] = br[d]
+    t0 = time.perf_counter()
+    oracle.rrr_transpose_tiles(tiles_c, inv, br_perm, threads)
+    t_untr = time.perf_counter() - t0
+    t_total = t_tr + t_pack + t_gemm + t_unpack + t_untr
     flops = 2.0 * M * N * K
     return {
         "value": round(flops / t_total / 1e12, 5),
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"full tile transposes ({t_tr:.2f}s) and big-matrix packs ({t_pack:.2f}s) of "
-                   f"A,B,C; blocked OpenMP dgemm on a {rows}x{N}x{K} row slab of the {M}x{N}x{K} "
-                   f"GEMM ({t_gemm_slab:.2f}s) extrapolated to all rows; C unpack estimated as "
-                   f"one third of transpose+pack"),
+        "host": core_desc,
+        "seconds": {"transpose": round(t_tr, 3), "pack": round(t_pack, 3),
+                    "dgemm_extrapolated": round(t_gemm, 3), "unpack_c": round(t_unpack, 3),
+                    "inverse_transpose_c": round(t_untr, 3)},
+        "sample": (f"full tile transposes and big-matrix packs of A, B, C; blocked OpenMP dgemm "
+                   f"on a {rows}x{N}x{K} row slab of the {M}x{N}x{K} GEMM ({t_gemm_slab:.2f}s) "
+                   f"extrapolated to all rows; full unpack and inverse transposes of C; "
+                   f"{threads} threads"),
     }
+
+
+def traffic_record(mode, o, v, live_ms):
+    """HBM bytes per launch of the step's GEMM from profiles/traffic.json
+    (FETCH_SIZE x fetch_scale + WRITE_SIZE x write_scale, KiB per dispatch)
+    for this workload, with the profiled kernel's average duration beside
+    the live one: a kernel that has changed since its profile shows as a
+    duration mismatch (traffic_stale)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            recs = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    for key, r in recs.items():
+        w = r.get("workload", {})
+        if (w.get("o"), w.get("v"), w.get("mode")) != (o, v, mode):
+            continue
+        traffic = 1024.0 * (r["fetch_kib_per_dispatch"] * r["fetch_scale"] +
+                            r["write_kib_per_dispatch"] * r["write_scale"])
+        prof_ms = r["avg_duration_us"] / 1e3
+        return {"traffic": round(traffic, -6), "traffic_source": f"profiles/traffic.json:{key}",
+                "traffic_profile_kernel_ms": round(prof_ms, 4),
+                "traffic_stale": abs(prof_ms - live_ms) > 0.05 * live_ms}
+    return {}
+
+
+def leg_ccsd(log, o=96, v=384, iters=3):
+    """Full UHF CCSD iterations (T1 + T2 residual programs, FP64 amplitude
+    update, singles-inclusive correlation energy): FP64 against mixed
+    residual programs from the same amplitudes.  The system is generated as
+    the precision study's (eps_occ ~ U[-2,-0.5], eps_vir ~ U[0.2,3],
+    antisymmetrised normal integrals) with sigma = 0.5 / v, a scale at
+    which the Jacobi iteration converges; with sigma = 0.05 it diverges, and
+    that case is reported for the first iteration from the first-order
+    amplitudes at o=64, v=256."""
+    from dtcf_amd import ccsd
+    res = ccsd.study(o, v, iters, modes=("fp64", "int8x4/L5", "int8x4/L4", "int8x3/L4"),
+                     sigma=0.5 / v, log=lambda s_: log("[ccsd] " + s_))
+    out = {"workload": f"UHF CCSD iterations o={o} v={v} (Oa=Ob={o // 2}, Va=Vb={v // 2}), "
+                       f"sigma={0.5 / v:.3g}, {iters} iterations from the first-order amplitudes, "
+                       f"every mode from the FP64 iteration's amplitudes",
+           "flops_per_iteration": res["flops_per_iteration"], "modes": {}}
+    for m, r in res["modes"].items():
+        out["modes"][m] = {"seconds_per_iteration": round(r["seconds_per_iteration"], 4),
+                           "tflops_residuals": round(r["tflops_residuals"], 2),
+                           "speedup": round(r["speedup"], 3),
+                           "energy": [round(x, 10) for x in r["energy"]],
+                           "max_energy_err": {k: sig(x) for k, x in r["max_energy_err"].items()},
+                           "max_residual_rel_err": {k: sig(x) for k, x in
+                                                    r["max_residual_rel_err"].items()}}
+    ok = [(m, r) for m, r in res["modes"].items()
+          if m != "fp64" and r["max_energy_err"]["E"] <= 1e-7]
+    if ok:
+        m, r = max(ok, key=lambda x: x[1]["speedup"])
+        out["best_within_1e-7_Eh"] = {"mode": m, "speedup": round(r["speedup"], 3),
+                                      "max_energy_err": sig(r["max_energy_err"]["E"])}
+    import torch
+    torch.cuda.empty_cache()
+    first = ccsd.study(64, 256, 1, modes=("fp64", "int8x5", "int8x4/L6", "int8x3/L4"), sigma=0.05,
+                       log=lambda s_: log("[ccsd sigma=0.05] " + s_))
+    out["sigma_0.05_first_iteration_o64v256"] = {
+        m: {"speedup": round(r["speedup"], 3), "energy": r["energy"][0],
+            "energy_err": sig(r["max_energy_err"]["E"])} for m, r in first["modes"].items()}
+    torch.cuda.empty_cache()
+    return out
 
 
 def leg_precision(log):
@@ -138,8 +263,12 @@ def leg_precision(log):
     res = precision.study(64, 256, variants=variants, reps=2, log=lambda s_: log("[precision] " + s_))
     keep = ("name", "seconds", "seconds_integrals_resident", "tflops", "energy_err", "max_rel_err",
             "speedup", "speedup_integrals_resident", "products")
+
+    def terms(x):
+        return {t: {k: sig(val) for k, val in d.items()} for t, d in x.get("terms", {}).items()}
+
     out = {"workload": "ladder + ring, o=64 v=256, one CCD-like iteration", "variants": [
-        {k: sig(x[k]) for k in keep} for x in res["variants"]]}
+        dict({k: sig(x[k]) for k in keep}, terms=terms(x)) for x in res["variants"]]}
     ok = [x for x in res["variants"] if x["name"] != "fp64" and x["energy_err"] <= 1e-7]
     if ok:
         b = max(ok, key=lambda x: x["speedup_integrals_resident"])
@@ -360,21 +489,18 @@ def main():
     M, N, Kd = best["big_mnk"]
     gemm_s = best["gemm_ms"] / 1e3
     achieved = 2.0 * M * N * Kd / gemm_s / 1e12
-    # HBM bytes per launch of the fused-mode GEMM, measured with rocprofv3
-    # --pmc FETCH_SIZE and WRITE_SIZE (separate runs) and scaled by a 2 GiB
-    # copy kernel's counts (fetch x2.0, write x1.0); profiles/
-    # gemm_f64_ring_o48_v192.md holds the runs.  Algorithmic: 2.72 GB.
-    traffic = TRAFFIC_BYTES.get((chosen, o, v))
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TF,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TF, 4), "traffic": traffic,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TF, 4), "traffic": None,
                 "kernel": "gemm_f64_kernel (FP64 MFMA 16x16x4)",
                 "kernel_ms": round(best["gemm_ms"], 4)}
+    roofline.update(traffic_record(chosen, o, v, best["gemm_ms"]))
 
     del tA, tB, tC, tens
     torch.cuda.empty_cache()
     extra = {}
     if not args.no_extra:
         for key, fn in (("c1", lambda: leg_c1(dev, log)),
+                        ("ccsd", lambda: leg_ccsd(log)),
                         ("precision", lambda: leg_precision(log)),
                         ("blocksparse", lambda: leg_blocksparse(dev, log)),
                         ("serving", lambda: leg_serving(log))):

@@ -218,6 +218,7 @@ MODES = {
     "int8x5": (mixed.I8, 5, 0),
     "int8x4/L6": (mixed.I8, 4, 6),
     "int8x4/L5": (mixed.I8, 4, 5),
+    "int8x4/L4": (mixed.I8, 4, 4),
     "int8x3/L4": (mixed.I8, 3, 4),
 }
 

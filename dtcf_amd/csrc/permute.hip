@@ -115,6 +115,94 @@ __global__ __launch_bounds__(256) void permute_kernel(const double *__restrict__
     }
 }
 
+// 16-byte forms of the two tiles, taken when the tile's fast strides are 1
+// on their side, the extents and every other offset even and the buffers
+// 16-byte aligned (no per-tile offset tables): TRANSPOSE 64 x 64 through
+// LDS, 16 elements per thread, reads two consecutive p and writes two
+// consecutive q per access; DIRECT 16 x 128, 8 elements per thread, q
+// contiguous on both sides.  Same values as permute_kernel.
+template <bool TRANSPOSE>
+__global__ __launch_bounds__(256) void permute_vec_kernel(const double *__restrict__ in,
+                                                          double *__restrict__ out, PermParams P)
+{
+    constexpr int TP = TRANSPOSE ? 64 : 16;
+    constexpr int TQ = TRANSPOSE ? 64 : 128;
+    __shared__ double lds[TRANSPOSE ? 64 : 1][TRANSPOSE ? 65 : 1];
+    const int tid = threadIdx.x;
+    for (uint64_t blk = blockIdx.x; blk < P.total; blk += gridDim.x) {
+        const uint64_t t = blk / P.per_tile;
+        uint32_t rem = (uint32_t)(blk - t * P.per_tile);
+        const uint32_t qt_div = P.div_ntq.div(rem);
+        const uint32_t qt = rem - qt_div * P.ntq;
+        rem = qt_div;
+        const uint32_t pt_div = P.div_ntp.div(rem);
+        const uint32_t pt = rem - pt_div * P.ntp;
+        rem = pt_div;
+        int64_t src = (int64_t)t * P.src_step, dst = (int64_t)t * P.dst_step;
+        for (int d = P.n_other - 1; d >= 0; d--) {
+            const uint32_t q = P.other_div[d].div(rem);
+            const uint32_t x = rem - q * P.other_ext[d];
+            rem = q;
+            src += (int64_t)x * P.other_istr[d];
+            dst += (int64_t)x * P.other_ostr[d];
+        }
+        const int64_t p0 = (int64_t)pt * TP, q0 = (int64_t)qt * TQ;
+        auto put = [&](double *o, double2 v) {
+            if (P.beta == 0.0) {
+                if (P.alpha != 1.0) {
+                    v.x *= P.alpha;
+                    v.y *= P.alpha;
+                }
+            } else {
+                const double2 w = *reinterpret_cast<const double2 *>(o);
+                v.x = P.alpha * v.x + P.beta * w.x;
+                v.y = P.alpha * v.y + P.beta * w.y;
+            }
+            *reinterpret_cast<double2 *>(o) = v;
+        };
+        if constexpr (TRANSPOSE) {
+            const int tx = tid & 31, ty = tid >> 5;
+            double2 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {  // rows q, pairs of p (unit input stride)
+                const int64_t pp = p0 + 2 * tx, qq = q0 + ty + 8 * i;
+                v[i] = (pp < P.ext_p && qq < P.ext_q)
+                           ? *reinterpret_cast<const double2 *>(in + src + pp + qq * P.istr_q)
+                           : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                lds[ty + 8 * i][2 * tx] = v[i].x;
+                lds[ty + 8 * i][2 * tx + 1] = v[i].y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 8; i++) {  // rows p, pairs of q (unit output stride)
+                const int64_t qq = q0 + 2 * tx, pp = p0 + ty + 8 * i;
+                if (pp < P.ext_p && qq < P.ext_q)
+                    put(out + dst + qq + pp * P.ostr_p,
+                        make_double2(lds[2 * tx][ty + 8 * i], lds[2 * tx + 1][ty + 8 * i]));
+            }
+            __syncthreads();
+        } else {
+            const int tx = tid & 63, ty = tid >> 6;
+            double2 v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t qq = q0 + 2 * tx, pp = p0 + ty + 4 * i;
+                v[i] = (pp < P.ext_p && qq < P.ext_q)
+                           ? *reinterpret_cast<const double2 *>(in + src + qq + pp * P.istr_p)
+                           : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t qq = q0 + 2 * tx, pp = p0 + ty + 4 * i;
+                if (pp < P.ext_p && qq < P.ext_q) put(out + dst + qq + pp * P.ostr_p, v[i]);
+            }
+        }
+    }
+}
+
 struct Dim {
     int64_t ext, istr, ostr;
 };
@@ -195,7 +283,17 @@ extern "C" int dtcf_permute_copy_f64(const double *in, double *out, const dtcf_s
     P.ostr_p = dp.ostr;
     P.istr_q = dq.istr;
     P.ostr_q = dq.ostr;
-    const int TP = transpose ? 32 : 16, TQ = transpose ? 32 : 64;
+    // 16-byte form: unit fast strides, even extents and offsets, aligned
+    // buffers, offsets from steps (not tables)
+    bool vec = !src_off && !dst_off && (reinterpret_cast<uintptr_t>(in) & 15) == 0 &&
+               (reinterpret_cast<uintptr_t>(out) & 15) == 0 && src_step % 2 == 0 &&
+               dst_step % 2 == 0 && dq.ext % 2 == 0 && dq.ostr == 1;
+    if (transpose) vec = vec && dp.istr == 1 && dp.ext % 2 == 0 && dq.istr % 2 == 0 && dp.ostr % 2 == 0;
+    else vec = vec && dq.istr == 1 && (p < 0 || (dp.istr % 2 == 0 && dp.ostr % 2 == 0));
+    for (int i = 0; vec && i < (int)dims.size(); i++)
+        if (i != q && i != p) vec = dims[i].istr % 2 == 0 && dims[i].ostr % 2 == 0;
+    const int TP = vec ? (transpose ? 64 : 16) : (transpose ? 32 : 16);
+    const int TQ = vec ? (transpose ? 64 : 128) : (transpose ? 32 : 64);
     const uint64_t ntp = (dp.ext + TP - 1) / TP, ntq = (dq.ext + TQ - 1) / TQ;
     uint64_t per = ntp * ntq;
     int n_other = 0;
@@ -224,7 +322,11 @@ extern "C" int dtcf_permute_copy_f64(const double *in, double *out, const dtcf_s
     const uint64_t max_grid = 256ull * 64;  // plenty of waves in flight per CU
     const unsigned grid = (unsigned)std::min<uint64_t>(P.total, max_grid);
     hipStream_t s = as_stream(stream);
-    if (transpose)
+    if (vec && transpose)
+        hipLaunchKernelGGL(permute_vec_kernel<true>, dim3(grid), dim3(256), 0, s, in, out, P);
+    else if (vec)
+        hipLaunchKernelGGL(permute_vec_kernel<false>, dim3(grid), dim3(256), 0, s, in, out, P);
+    else if (transpose)
         hipLaunchKernelGGL(permute_kernel<true>, dim3(grid), dim3(256), 0, s, in, out, P, src_off,
                            dst_off);
     else

@@ -351,8 +351,10 @@ def test_local_step_symmetric_bounce_literal(dev):
         rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=rrr.FUSED)
 
 
-def test_reductions_vs_oracle(dev):
-    o, v = 6, 10
+@pytest.mark.parametrize("o,v", [(6, 10), (5, 7), (12, 70)])
+def test_reductions_vs_oracle(dev, o, v):
+    # (6, 10) and (12, 70): 16-byte paths, several 64 x 64 energy tiles;
+    # (5, 7): odd o*o and v*v, the element paths
     eo = -2.0 + 1.5 * RNG.random(o)
     ev = 0.2 + 2.8 * RNG.random(v)
     R = RNG.standard_normal(v * v * o * o) * 0.05
