@@ -253,6 +253,22 @@ def leg_ccsd(log, o=96, v=384, iters=3):
     return out
 
 
+def leg_dropin(log):
+    """The drop-in entry points at the reference's call shapes, from C++
+    through the exported symbols (dtcf_amd/dropin_bench, built with the
+    libraries): local_contraction at 9216^3 with host operands, and the o=8,
+    v=32 ladder's 1024 tile tasks issued as the reference's task_mul does
+    (host tiles) and as TaskFunc tasks over device tensors."""
+    import subprocess
+    exe = os.path.join(ROOT, "dtcf_amd", "dropin_bench")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"dropin_bench failed ({r.returncode}): {r.stdout[-300:]} {r.stderr[-300:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    log(f"[dropin] {json.dumps(out)}")
+    return out
+
+
 def leg_precision(log):
     """Precision study (ladder + ring, o=64, v=256): FP64 against the mixed
     modes that bracket the 1e-7 Eh target."""
@@ -500,6 +516,7 @@ def main():
     extra = {}
     if not args.no_extra:
         for key, fn in (("c1", lambda: leg_c1(dev, log)),
+                        ("dropin", lambda: leg_dropin(log)),
                         ("ccsd", lambda: leg_ccsd(log)),
                         ("precision", lambda: leg_precision(log)),
                         ("blocksparse", lambda: leg_blocksparse(dev, log)),

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -62,6 +63,11 @@ int dtcf_init(int device)
     DTCF_HIP(hipSetDevice(device));
     DTCF_HIP(hipFree(nullptr));
     return ccsd_scratch_setup(device);
+}
+
+int dtcf_gemm_reserve_workspace(void *stream)
+{
+    return gemm_reserve_workspace(as_stream(stream));
 }
 
 int dtcf_gemm_set_config(int cfg)
@@ -469,13 +475,108 @@ extern "C" int dtcf_view_scatter_f64(const double *src, int64_t ld, dtcf_view v,
 }
 
 // ---------------------------------------------------------------------------
-// Host-pointer variants: stage through device buffers on a private stream.
+// Host-pointer variants.  Per device: device buffers kept between calls
+// (grown on demand, never freed), two pinned host staging buffers and a
+// private stream.  Host memory moves through the pinned buffers in chunks:
+// the CPU fills (or drains) one while the DMA engine moves the other, so a
+// call costs one pass of host memcpy overlapped with the PCIe transfer and
+// no allocation once the buffers have grown.  Calls on one device are
+// serialised (the reference calls these from one host thread per rank).
 // ---------------------------------------------------------------------------
 namespace {
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+constexpr size_t kStageChunk = size_t(64) << 20;  // bytes per pinned buffer
+
+struct HostStage {
+    std::mutex mu;
+    bool ready = false;
+    hipStream_t s = nullptr;
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void *dev[3] = {nullptr, nullptr, nullptr};
+    size_t cap[3] = {0, 0, 0};
+
+    int init()
+    {
+        if (ready) return DTCF_OK;
+        DTCF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        for (int b = 0; b < 2; b++) {
+            DTCF_HIP(hipHostMalloc(&pin[b], kStageChunk, hipHostMallocDefault));
+            DTCF_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+            DTCF_HIP(hipEventRecord(ev[b], s));  // recorded once: waiting on it is always defined
+        }
+        ready = true;
+        return DTCF_OK;
+    }
+    // device buffer `slot` of at least `bytes`
+    int buf(int slot, size_t bytes, void **p)
+    {
+        if (cap[slot] < bytes) {
+            if (dev[slot]) {
+                DTCF_HIP(hipStreamSynchronize(s));
+                DTCF_HIP(hipFree(dev[slot]));
+                dev[slot] = nullptr;
+                cap[slot] = 0;
+            }
+            DTCF_HIP(hipMalloc(&dev[slot], bytes));
+            cap[slot] = bytes;
+        }
+        *p = dev[slot];
+        return DTCF_OK;
+    }
+    int h2d(void *dst, const void *src, size_t bytes)
+    {
+        const char *h = static_cast<const char *>(src);
+        char *d = static_cast<char *>(dst);
+        for (size_t off = 0, c = 0; off < bytes; off += kStageChunk, c++) {
+            const int b = (int)(c & 1);
+            const size_t len = std::min(kStageChunk, bytes - off);
+            DTCF_HIP(hipEventSynchronize(ev[b]));  // every earlier DMA from it is done
+            std::memcpy(pin[b], h + off, len);
+            DTCF_HIP(hipMemcpyAsync(d + off, pin[b], len, hipMemcpyHostToDevice, s));
+            DTCF_HIP(hipEventRecord(ev[b], s));
+        }
+        return DTCF_OK;
+    }
+    int d2h(void *dst, const void *src, size_t bytes)
+    {
+        char *h = static_cast<char *>(dst);
+        const char *d = static_cast<const char *>(src);
+        size_t prev_off = 0, prev_len = 0;
+        size_t c = 0;
+        for (size_t off = 0; off < bytes; off += kStageChunk, c++) {
+            const int b = (int)(c & 1);
+            const size_t len = std::min(kStageChunk, bytes - off);
+            DTCF_HIP(hipMemcpyAsync(pin[b], d + off, len, hipMemcpyDeviceToHost, s));
+            DTCF_HIP(hipEventRecord(ev[b], s));
+            if (c >= 1) {  // drain the previous chunk while this one moves
+                DTCF_HIP(hipEventSynchronize(ev[b ^ 1]));
+                std::memcpy(h + prev_off, pin[b ^ 1], prev_len);
+            }
+            prev_off = off;
+            prev_len = len;
+        }
+        if (c >= 1) {
+            const int b = (int)((c - 1) & 1);
+            DTCF_HIP(hipEventSynchronize(ev[b]));
+            std::memcpy(h + prev_off, pin[b], prev_len);
+        }
+        return DTCF_OK;
+    }
 };
+
+std::mutex g_stage_mu;
+std::map<int, HostStage *> g_stage;
+
+int host_stage(HostStage **out)
+{
+    int dev = 0;
+    DTCF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    auto &p = g_stage[dev];
+    if (!p) p = new HostStage();  // lives until the process ends
+    *out = p;
+    return DTCF_OK;
+}
 }  // namespace
 
 extern "C" int dtcf_gemm_f64_host(int transa, int transb, int64_t m, int64_t n, int64_t k,
@@ -488,18 +589,21 @@ extern "C" int dtcf_gemm_f64_host(int transa, int transb, int64_t m, int64_t n, 
     const int64_t asz = transa ? (k - 1) * lda + m : (m - 1) * lda + k;
     const int64_t bsz = transb ? (n - 1) * ldb + k : (k - 1) * ldb + n;
     const int64_t csz = (m - 1) * ldc + n;
-    DevBuf da, db, dc;
-    DTCF_HIP(hipMalloc(&da.p, asz * sizeof(double)));
-    DTCF_HIP(hipMalloc(&db.p, bsz * sizeof(double)));
-    DTCF_HIP(hipMalloc(&dc.p, csz * sizeof(double)));
-    DTCF_HIP(hipMemcpy(da.p, A, asz * sizeof(double), hipMemcpyHostToDevice));
-    DTCF_HIP(hipMemcpy(db.p, B, bsz * sizeof(double), hipMemcpyHostToDevice));
-    DTCF_HIP(hipMemcpy(dc.p, C, csz * sizeof(double), hipMemcpyHostToDevice));
-    int st = dtcf_gemm_f64(transa, transb, m, n, k, alpha, (const double *)da.p, lda,
-                           (const double *)db.p, ldb, (double *)dc.p, ldc, nullptr);
+    HostStage *hs = nullptr;
+    int st = host_stage(&hs);
     if (st != DTCF_OK) return st;
-    DTCF_HIP(hipMemcpy(C, dc.p, csz * sizeof(double), hipMemcpyDeviceToHost));
-    return DTCF_OK;
+    std::lock_guard<std::mutex> lk(hs->mu);
+    void *da, *db, *dc;
+    if ((st = hs->init()) || (st = hs->buf(0, asz * sizeof(double), &da)) ||
+        (st = hs->buf(1, bsz * sizeof(double), &db)) || (st = hs->buf(2, csz * sizeof(double), &dc)))
+        return st;
+    if ((st = hs->h2d(da, A, asz * sizeof(double))) || (st = hs->h2d(db, B, bsz * sizeof(double))) ||
+        (st = hs->h2d(dc, C, csz * sizeof(double))))
+        return st;
+    st = dtcf_gemm_f64(transa, transb, m, n, k, alpha, (const double *)da, lda, (const double *)db,
+                       ldb, (double *)dc, ldc, hs->s);
+    if (st != DTCF_OK) return st;
+    return hs->d2h(C, dc, csz * sizeof(double));
 }
 
 extern "C" int dtcf_permute_tiles_f64_host(const double *in, double *out, int dims,
@@ -515,15 +619,18 @@ extern "C" int dtcf_permute_tiles_f64_host(const double *in, double *out, int di
     const int64_t total = bs * ntiles;
     if (total == 0) return DTCF_OK;
     DTCF_REQUIRE(in && out, "permute: null buffer");
-    DevBuf di, dout;
-    DTCF_HIP(hipMalloc(&di.p, total * sizeof(double)));
-    DTCF_HIP(hipMalloc(&dout.p, total * sizeof(double)));
-    DTCF_HIP(hipMemcpy(di.p, in, total * sizeof(double), hipMemcpyHostToDevice));
-    int st = dtcf_permute_tiles_f64((const double *)di.p, (double *)dout.p, dims, w, out_src,
-                                    ntiles, nullptr);
+    HostStage *hs = nullptr;
+    int st = host_stage(&hs);
     if (st != DTCF_OK) return st;
-    DTCF_HIP(hipMemcpy(out, dout.p, total * sizeof(double), hipMemcpyDeviceToHost));
-    return DTCF_OK;
+    std::lock_guard<std::mutex> lk(hs->mu);
+    void *di, *dout;
+    if ((st = hs->init()) || (st = hs->buf(0, total * sizeof(double), &di)) ||
+        (st = hs->buf(1, total * sizeof(double), &dout)))
+        return st;
+    if ((st = hs->h2d(di, in, total * sizeof(double)))) return st;
+    st = dtcf_permute_tiles_f64((const double *)di, (double *)dout, dims, w, out_src, ntiles, hs->s);
+    if (st != DTCF_OK) return st;
+    return hs->d2h(out, dout, total * sizeof(double));
 }
 
 extern "C" int dtcf_pack_blocks_f64_host(double *tiles, int64_t ntiles, const int64_t *order,
@@ -541,27 +648,30 @@ extern "C" int dtcf_pack_blocks_f64_host(double *tiles, int64_t ntiles, const in
         DTCF_REQUIRE(order[b] >= 0 && order[b] < ntiles, "pack: tile number out of range");
         blk[b] = order[b] * bs;
     }
-    dtcf_view v = nullptr;
-    int st = dtcf_view_create(rows, cols, rofs.data(), cofs.data(), blk.data(), rpb, cpb, &v);
+    HostStage *hs = nullptr;
+    int st = host_stage(&hs);
     if (st != DTCF_OK) return st;
-    DevBuf dt, dbig;
-    hipError_t e = hipMalloc(&dt.p, ntiles * bs * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&dbig.p, rows * cols * sizeof(double));
-    if (e == hipSuccess)
-        e = unpack ? hipMemcpy(dbig.p, big, rows * cols * sizeof(double), hipMemcpyHostToDevice)
-                   : hipMemcpy(dt.p, tiles, ntiles * bs * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess && unpack)  // tiles outside the order keep their values
-        e = hipMemcpy(dt.p, tiles, ntiles * bs * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        dtcf_view_destroy(v);
-        return hip_fail(e, "pack: staging");
-    }
-    st = unpack ? dtcf_view_scatter_f64((const double *)dbig.p, cols, v, (double *)dt.p, nullptr)
-                : dtcf_view_gather_f64((const double *)dt.p, v, (double *)dbig.p, cols, nullptr);
+    std::lock_guard<std::mutex> lk(hs->mu);
+    void *dt, *dbig;
+    if ((st = hs->init()) || (st = hs->buf(0, ntiles * bs * sizeof(double), &dt)) ||
+        (st = hs->buf(1, rows * cols * sizeof(double), &dbig)))
+        return st;
+    dtcf_view v = nullptr;
+    st = dtcf_view_create(rows, cols, rofs.data(), cofs.data(), blk.data(), rpb, cpb, &v);
+    if (st != DTCF_OK) return st;
+    if (unpack) st = hs->h2d(dbig, big, rows * cols * sizeof(double));
+    // unpack: tiles outside the order keep their values
+    if (st == DTCF_OK) st = hs->h2d(dt, tiles, ntiles * bs * sizeof(double));
+    if (st == DTCF_OK)
+        st = unpack ? dtcf_view_scatter_f64((const double *)dbig, cols, v, (double *)dt, hs->s)
+                    : dtcf_view_gather_f64((const double *)dt, v, (double *)dbig, cols, hs->s);
+    if (st == DTCF_OK)
+        st = unpack ? hs->d2h(tiles, dt, ntiles * bs * sizeof(double))
+                    : hs->d2h(big, dbig, rows * cols * sizeof(double));
+    // the view's tables are read by the kernel: release them once it is done
+    const hipError_t e = hipStreamSynchronize(hs->s);
     dtcf_view_destroy(v);
     if (st != DTCF_OK) return st;
-    e = unpack ? hipMemcpy(tiles, dt.p, ntiles * bs * sizeof(double), hipMemcpyDeviceToHost)
-               : hipMemcpy(big, dbig.p, rows * cols * sizeof(double), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return hip_fail(e, "pack: copy back");
+    if (e != hipSuccess) return hip_fail(e, "pack: stream");
     return DTCF_OK;
 }

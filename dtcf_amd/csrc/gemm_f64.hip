@@ -491,6 +491,16 @@ int split_workspace(hipStream_t s, size_t need, double **ws)
     std::lock_guard<std::mutex> lk(g_dev_mu);
     auto &e = g_ws[std::make_pair(d, s)];
     if (e.second < need) {
+        // growing allocates (and frees the old buffer): never inside a graph
+        // capture, where the allocation would run outside the graph and the
+        // free could release a buffer an earlier graph still uses
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        DTCF_HIP(hipStreamIsCapturing(s, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            set_error("gemm: the split-K workspace of a capturing stream must be reserved before "
+                      "capture (dtcf_gemm_reserve_workspace)");
+            return DTCF_ERR_INVALID;
+        }
         if (e.first) DTCF_HIP(hipStreamSynchronize(s));  // the old buffer may be in use
         if (e.first) (void)hipFree(e.first);
         e.first = nullptr;

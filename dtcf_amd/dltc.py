@@ -63,6 +63,7 @@ def _declare():
         "dtcf_dltc_transpose_f64": [vp, C.POINTER(C.c_int32), vp, C.POINTER(C.c_int32), C.c_int,
                                     vp, vp],
         "dtcf_view_sub": [vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.POINTER(vp)],
+        "dtcf_gemm_reserve_workspace": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -131,12 +132,18 @@ class TaskGraph:
     the same buffers, which must stay allocated and in place.  Per output
     tile the order of the tasks is the eager one, so results are bitwise
     those of issuing the chains one after the other.  Call each entry point
-    once eagerly before capturing (first calls set kernel attributes)."""
+    once eagerly before capturing (first calls set kernel attributes).  A
+    GEMM that splits K uses its stream's workspace, which cannot grow under
+    capture: every branch stream's workspace is reserved first (at the bound
+    any GEMM can need), so the graph never allocates and later growth on a
+    reused stream cannot free a buffer the graph holds."""
 
     def __init__(self, chains, branches=4):
         import torch
         self.graph = torch.cuda.CUDAGraph()
         self.streams = [torch.cuda.Stream() for _ in range(max(1, branches))]
+        for st in self.streams:
+            check(lib().dtcf_gemm_reserve_workspace(C.c_void_p(st.cuda_stream)))
         with torch.cuda.graph(self.graph):
             cur = torch.cuda.current_stream()
             fork = torch.cuda.Event()

@@ -131,6 +131,15 @@ int dtcf_view_gather_f64(const double *src, dtcf_view v, double *dst, int64_t ld
  * dst[view(r, c)] = src[r*ld + c]. */
 int dtcf_view_scatter_f64(const double *src, int64_t ld, dtcf_view v, double *dst, void *stream);
 
+/* GEMMs that split K keep their partial sums in a workspace of the stream
+ * they run on, grown on demand (an allocation).  Under HIP graph capture it
+ * cannot grow: such a GEMM returns DTCF_ERR_INVALID.  This call grows the
+ * stream's workspace now to the bound any GEMM launch on this device can
+ * need (151 MB on 256 CUs), so captures on that stream never allocate;
+ * call it, outside capture, for every stream a graph will record GEMMs on.
+ * No counterpart in the reference (its dgemm runs on the host). */
+int dtcf_gemm_reserve_workspace(void *stream);
+
 /* Tile-shape selection for the GEMM kernels: 0 = 128x128, 1 = 192x96,
  * 2 = 96x192, 3 = 64x64, -1 = automatic (autotuned table). */
 int dtcf_gemm_set_config(int cfg);
