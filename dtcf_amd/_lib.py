@@ -1,8 +1,8 @@
 """ctypes binding of libdtcf.so (the C ABI in include/dtcf_local.h).
 
-The library is built in-tree by ``dtcf_amd.build`` (hipcc, gfx950).  If it
-is missing, importing the binding raises: there is no CPU fallback anywhere
-in the product path.
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C
+dtcf_amd/csrc``: hipcc, gfx950).  If it is missing, importing the binding
+raises: there is no CPU fallback anywhere in the product path.
 """
 from __future__ import annotations
 

@@ -388,6 +388,7 @@ struct dtcf_server_s {
     int64_t next_id = 1, next_seq = 1;
     std::chrono::steady_clock::time_point t0;
     std::string fatal;
+    int64_t inject = -1;  // statements to issue before an injected failure (-1: none)
 };
 
 namespace {
@@ -777,7 +778,15 @@ bool sched_pass(dtcf_server srv)
             continue;
         }
         r->t_issue[i] = now_us(srv->t0);
-        int st = launch_stmt(srv, r, i, sl.s);
+        int st;
+        if (srv->inject == 0) {  // host-forced failure: nothing is launched
+            srv->inject = -1;
+            set_error("injected statement failure");
+            st = DTCF_ERR_HIP;
+        } else {
+            if (srv->inject > 0) srv->inject--;
+            st = launch_stmt(srv, r, i, sl.s);
+        }
         hipError_t e = st == DTCF_OK ? hipEventRecord(sl.ev, sl.s) : hipSuccess;
         if (st != DTCF_OK || e != hipSuccess) {
             r->status = st != DTCF_OK ? st : DTCF_ERR_HIP;
@@ -919,6 +928,14 @@ extern "C" int dtcf_server_free(dtcf_server s, void *ptr)
     s->arena.release(it->second.first, it->second.second);
     s->resident -= granule(it->second.second);
     s->tensors.erase(it);
+    return DTCF_OK;
+}
+
+extern "C" int dtcf_server_inject_failure(dtcf_server s, int64_t after)
+{
+    DTCF_REQUIRE(s && after >= -1, "server_inject_failure: bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->inject = after;
     return DTCF_OK;
 }
 

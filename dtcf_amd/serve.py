@@ -39,6 +39,7 @@ def _declare():
         "dtcf_server_alloc": [vp, i64, C.POINTER(vp)],
         "dtcf_server_free": [vp, vp],
         "dtcf_server_usage": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
+        "dtcf_server_inject_failure": [vp, i64],
         "dtcf_program_create": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)],
         "dtcf_program_array": [vp, C.c_char_p, C.c_int, C.POINTER(i64), C.c_int],
         "dtcf_program_bind": [vp, C.c_char_p, vp],
@@ -131,6 +132,10 @@ class Server:
         check(lib().dtcf_server_usage(self.h, *[C.byref(x) for x in v]))
         return dict(budget=v[0].value, resident=v[1].value, reserved=v[2].value,
                     peak_reserved=v[3].value)
+
+    def inject_failure(self, after):
+        """The statement issued after `after` more fails on the host (tests)."""
+        check(lib().dtcf_server_inject_failure(self.h, int(after)))
 
     def wait(self, rid, timeout_ms=-1.0):
         return lib().dtcf_request_wait(self.h, int(rid), float(timeout_ms))

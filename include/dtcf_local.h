@@ -497,6 +497,12 @@ int dtcf_server_free(dtcf_server s, void *dev_ptr);
  * requests; peak_reserved: high-water mark of the request reservations. */
 int dtcf_server_usage(dtcf_server s, int64_t *budget, int64_t *resident, int64_t *reserved,
                       int64_t *peak_reserved);
+/* Failure injection (testing the failure path; no reference counterpart):
+ * after `after` more statements have been issued, the next one fails on the
+ * host with DTCF_ERR_HIP without launching anything; its request issues
+ * nothing more, finishes once its statements in flight complete, reports
+ * the failure from dtcf_request_wait and releases its block.  -1 cancels. */
+int dtcf_server_inject_failure(dtcf_server s, int64_t after);
 
 /* A program is the statement list of one term file over one system's
  * extents (a CC residual evaluation).  Arrays are dense row-major FP64.

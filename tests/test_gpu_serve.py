@@ -302,3 +302,44 @@ def test_auto_layout_choice(dev, lists):
         assert auto.layout == S.LAYOUT_SEPARATE and auto.block_bytes == sep.block_bytes
         for p in (auto, reuse, sep):
             p.destroy()
+
+
+@pytest.mark.parametrize("nslots", [1, 8])
+def test_failed_statement_fails_its_request_only(dev, lists, nslots):
+    """A statement that fails at issue (forced on the host: nothing faults)
+    fails its request -- dtcf_request_wait reports DTCF_ERR_HIP with the
+    message, the request is marked failed and its block is released --
+    while a request submitted beside it completes with the result it has
+    alone, and the server serves the program again afterwards."""
+    tl = lists["t2"]
+    sizes = {"Oa": 4, "Ob": 3, "Va": 10, "Vb": 8}
+    with S.Server(2 << 30, nslots=nslots) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=sizes, seed=21)
+        p = S.Program(srv, tl, sysm)
+        good = p.run()
+        torch.cuda.synchronize()
+        srv.inject_failure(40)  # the 41st statement issued from now on
+        rid_bad, _ = p.submit()
+        st = srv.wait(rid_bad, 120000)
+        assert st == -3, st  # DTCF_ERR_HIP
+        assert b"injected" in lib().dtcf_last_error()
+        inf = srv.info(rid_bad)
+        assert inf.state == 3
+        srv.release(rid_bad)
+        assert srv.usage()["reserved"] == 0
+        # a failure in a mix: the other request is unaffected
+        srv.inject_failure(25)
+        rid_a, outs_a = p.submit()
+        rid_b, outs_b = p.submit()
+        sts = [srv.wait(r, 120000) for r in (rid_a, rid_b)]
+        assert sorted(sts) == [-3, 0], sts
+        ok_outs = outs_a if sts[0] == 0 else outs_b
+        for name, o, shp in zip(p.outputs, ok_outs, p.out_shapes):
+            assert np.array_equal(o.cpu().numpy().reshape(shp), good[name]), name
+        for r in (rid_a, rid_b):
+            srv.release(r)
+        assert srv.usage()["reserved"] == 0
+        again = p.run()
+        for name in good:
+            assert np.array_equal(again[name], good[name]), name
+        p.destroy()
