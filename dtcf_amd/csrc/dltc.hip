@@ -96,7 +96,8 @@ std::vector<int64_t> tile_major_offsets(const Space &S, const std::vector<int> &
 }
 
 // Maximal runs [first, last] of consecutive tile ranks inside a box of tile
-// ranges over `ids` (rank = row-major over the tile counts).
+// ranges over `ids` (rank = row-major over the tile counts).  An empty id
+// list (a term with no m, n or k index) has the one rank 0.
 std::vector<std::pair<int64_t, int64_t>> tile_runs(const Space &S, const std::vector<int> &ids,
                                                    const int32_t *lb, const int32_t *ub)
 {
@@ -119,8 +120,7 @@ std::vector<std::pair<int64_t, int64_t>> tile_runs(const Space &S, const std::ve
         }
         if (i < 0) break;
     }
-    if (n == 0) runs.push_back({0, 0});
-    return runs;
+    return runs;  // no ids: the single rank 0 (one run), as the loop leaves it
 }
 
 }  // namespace

@@ -190,14 +190,50 @@ def study(o, v, variants=VARIANTS, reps=1, log=print):
     return out
 
 
+def reductions(o, v, reps=5, log=print):
+    """The FP64 streaming kernels of an iteration at [v,v,o,o] arrays:
+    denominator update (R read, T written: 16 B per element), energy
+    (V_oovv and T read: 16 B) and the residual norm (R read: 8 B); each
+    timed with events over `reps` calls.  Returns GB/s per kernel."""
+    n = v * v * o * o
+    dev = "cuda"
+    R = mixed.fill_normal(n, 1, 0.05, dev)
+    Vm = mixed.fill_normal(n, 2, 0.05, dev)
+    eo = mixed.fill_uniform(o, 3, -2.0, -0.5, dev)
+    ev = mixed.fill_uniform(v, 4, 0.2, 3.0, dev)
+    T = torch.empty_like(R)
+    out = torch.zeros(1, dtype=torch.float64, device=dev)
+    res = {}
+    for name, fn, nbytes in (("denom_update", lambda: ops.denom_update(R, eo, ev, T), 16.0 * n),
+                             ("energy", lambda: ops.energy(Vm, T, o, v, out), 16.0 * n),
+                             ("norm2", lambda: ops.norm2(R, out), 8.0 * n)):
+        fn()
+        _sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        _sync()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": ms, "GB/s": nbytes / ms / 1e6, "bytes": nbytes}
+        log(f"{name}: {ms:.3f} ms, {nbytes / ms / 1e6:.0f} GB/s ({nbytes / 1e9:.2f} GB)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--o", type=int, default=64)
     ap.add_argument("--v", type=int, default=256)
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--reductions", action="store_true",
+                    help="time the denominator update, energy and norm kernels only")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     check(lib().dtcf_init(0))
+    if args.reductions:
+        reductions(args.o, args.v, max(args.reps, 5), log=lambda s: print(s, flush=True))
+        return 0
     res = study(args.o, args.v, reps=args.reps, log=lambda s: print(s, flush=True))
     if args.out:
         with open(args.out, "w") as f:

@@ -232,6 +232,36 @@ def test_task_graph_with_split_k_tasks(dev):
         assert torch.equal(Rg.cpu(), ref)
 
 
+def test_task_with_empty_index_set(dev):
+    """A term whose A operand is all summed (m has no index):
+    R[p2,h1] += -1 * T[p3,h4] * V[h4,p2,h1,p3] (DLTC/ccsd-t1.cc, m[0]).  Its
+    tile tasks run each GEMM once: the sum over every (ext, sum) tile equals
+    the whole term."""
+    import itertools
+    O, V, wbO, wbV = 4, 8, 2, 4
+    rng = np.random.default_rng(13)
+    h = {f"h{i}": i - 1 for i in range(1, 13)}
+    p = {f"p{i}": 11 + i for i in range(1, 13)}
+    T = rng.standard_normal((V, O))
+    W = rng.standard_normal((O, V, O, V))
+    R0 = rng.standard_normal((V, O))
+    c_ids, a_ids, b_ids = [p["p2"], h["h1"]], [p["p3"], h["h4"]], [h["h4"], p["p2"], h["h1"], p["p3"]]
+    sp = dltc.make_space(O, V, wbO, wbV)
+    Rd = dev_t(R0.copy().ravel(), dev)
+    term = dltc.Term(sp, c_ids, Rd, a_ids, dev_t(T, dev), b_ids, dev_t(W, dev), -1.0)
+    assert term.m_ids == []
+    nt = {p["p2"]: V // wbV, h["h1"]: O // wbO, p["p3"]: V // wbV, h["h4"]: O // wbO}
+    for e in itertools.product(range(nt[p["p2"]]), range(nt[h["h1"]])):
+        for s_ in itertools.product(range(nt[p["p3"]]), range(nt[h["h4"]])):
+            lb, ub = [0] * 24, [0] * 24
+            for x, val in zip([p["p2"], h["h1"], p["p3"], h["h4"]], e + s_):
+                lb[x], ub[x] = val, val + 1
+            term.task(lb, ub)
+    got = Rd.cpu().numpy().reshape(V, O)
+    ref = R0 - np.einsum("ck,kbjc->bj", T, W)
+    assert relerr(got - R0, ref - R0) <= TOL
+
+
 def test_task_mul_matches_tile_restatement(dev):
     # an uneven ext-tile box vs the oracle's tile-by-tile task_mul
     O, V, wbO, wbV = 6, 8, 2, 4
