@@ -481,10 +481,14 @@ static int permute_operand(dtcf_step s, OperandPlan &P, size_t off_base, bool in
             d.istr[i] = inverse ? ostr[comp[i]] : istr[i];
             d.ostr[i] = inverse ? istr[i] : ostr[comp[i]];
         }
-        const int64_t *offs = s->cls_off[off_base + c];
-        int r = inverse ? dtcf_permute_copy_f64(s->tmp, P.tiles, &d, n, offs, 0, offs, 0, 1.0,
+        // one class holding every tile: consecutive tiles, a step instead of
+        // an offset table (which lets the permute take its 16-byte form)
+        const bool all = n == P.ntiles;
+        const int64_t *offs = all ? nullptr : s->cls_off[off_base + c];
+        const int64_t step = all ? P.bs : 0;
+        int r = inverse ? dtcf_permute_copy_f64(s->tmp, P.tiles, &d, n, offs, step, offs, step, 1.0,
                                                 0.0, st)
-                        : dtcf_permute_copy_f64(P.tiles, s->tmp, &d, n, offs, 0, offs, 0, 1.0,
+                        : dtcf_permute_copy_f64(P.tiles, s->tmp, &d, n, offs, step, offs, step, 1.0,
                                                 0.0, st);
         if (r != DTCF_OK) return r;
     }
