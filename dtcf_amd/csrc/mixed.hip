@@ -27,6 +27,8 @@
 #include <cmath>
 #include <vector>
 
+#include <climits>
+
 #include "common.hpp"
 #include "gemm_f64.hpp"
 #include "view.hpp"
@@ -203,173 +205,215 @@ __global__ __launch_bounds__(256) void split_kernel(const double *__restrict__ s
     }
 }
 
-// INT8 split of a view without a block table whose column pairs are
-// contiguous and 16-byte aligned (colpairs).  A workgroup owns one
-// block of 16 rows (the rows of one fragment), wave w rows 4w .. 4w+3.
-// Every load instruction reads 256 elements of one row (2 KiB when the
-// row is contiguous);
-// per chunk of 256 elements (4 K steps) each wave turns its four rows into
-// digits, four elements per lane, and drops the 4-byte words into an LDS
-// copy of the chunk's fragments; the workgroup then writes each plane's
-// four fragments (4 KiB, contiguous in the fragment-major layout) with
-// 16-byte stores.  Same digits as split_kernel.
-__global__ __launch_bounds__(256) void split_i8_rows_kernel(const double *__restrict__ src, ViewArgs v,
-                                                           int64_t rows, int64_t K, int64_t rows_p,
-                                                           int64_t K_p, int nslice,
-                                                           int8_t *__restrict__ dst,
-                                                           int32_t *__restrict__ rexp)
+// INT8 split of a table view (no block table) in two K-parallel passes.
+// A row-parallel kernel that gives one workgroup all of K of a 16-row block
+// leaves most of the GPU idle on an operand of a few thousand rows; here
+// every workgroup takes a 16-row block by a slice of K:
+//   i8_exp_kernel: 16 rows x 1024 columns; the largest frexp exponent of
+//     its elements per row, merged into rexp[row] by an integer atomic max
+//     (order-independent; the same exponent as frexp of the row maximum).
+//     rexp starts at INT_MIN (i8_exp_init); a row that stays there is all
+//     zeros and reads as exponent 0.  Eight loads in flight per lane.
+//   i8_digit_kernel: 16 rows x 4 K steps, one per wave; lane l = r + 16 kq
+//     converts row r's elements 16 kq .. 16 kq + 15 of its K step (its own
+//     lane of the fragment) and stores 16 bytes per plane.
+// PAIRS views (column pairs contiguous and 16-byte aligned) read rows along
+// K; a wave stages its K step (16 rows x 64 columns) through LDS with
+// 16-byte loads of 2 rows x 512 contiguous bytes each.  Other views (RUN)
+// read per lane, 16 consecutive rows per load when rows run contiguously
+// in 16s (rowrun16), a gather otherwise.  The source is read once per pass
+// and the planes written once: both stream past the caches (nontemporal).
+// Same digits as split_kernel, bit for bit.
+__device__ __forceinline__ double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ d2 ld_nt2(const double *p) { return __builtin_nontemporal_load((const d2 *)p); }
+__device__ __forceinline__ void st_nt(uint4 *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
 {
-    __shared__ uint4 stage[6 * 256];  // [plane][K step of the chunk][fragment lane]
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t rb = blockIdx.x;
-    const double *xr[4];
-    bool real[4];
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    const u4v w = {a, b, c, d};
+    __builtin_nontemporal_store(w, (u4v *)p);
+}
+
+// Digits of 16 elements by integer byte extraction: with X = rint(x 2^sh)
+// (|X| <= 2^(8 NS - 2)) and B = sum_{s < NS} 128 * 256^s, U = X + B is an
+// integer in [0, 256^NS), exact in FP64, whose base-256 digits are the
+// balanced digits of X plus 128 (the balanced representation is unique).
+// Digit plane p (p = 0 the most significant) of element j is byte NS-1-p
+// of U, xor 0x80; wd[p][k] packs elements 4k .. 4k+3.
+template <int NS>
+__device__ __forceinline__ void i8_digit_words(const double (&xs)[16], int sh, uint32_t (&wd)[6][4])
+{
+    double B = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int64_t row = rb * 16 + 4 * w + i;
-        real[i] = row < rows;
-        xr[i] = real[i] ? src + v.rofs[row] : src;
-    }
-    // row maxima: lane covers elements 4 lane + 256 t .. +3 of each row
-    // (the pairs at cofs[c] and cofs[c + 2])
-    double m[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t c = 4 * lane; c < K; c += 256) {  // K is even: c + 1 < K
-        const int64_t o0 = v.cofs[c], o2 = c + 2 < K ? v.cofs[c + 2] : 0;
+    for (int s = 0; s < NS; s++) B = B * 256.0 + 128.0;
+    uint32_t lo[16], hi[16];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            if (!real[i]) continue;
-            const d2 a = *(const d2 *)(xr[i] + o0);
-            double t = fmax(fabs(a.x), fabs(a.y));
-            if (c + 2 < K) {
-                const d2 b = *(const d2 *)(xr[i] + o2);
-                t = fmax(t, fmax(fabs(b.x), fabs(b.y)));
-            }
-            m[i] = fmax(m[i], t);
+    for (int j = 0; j < 16; j++) {
+        const double U = rint(ldexp(xs[j], sh)) + B;
+        if constexpr (NS <= 4) {
+            lo[j] = (uint32_t)U;
+            hi[j] = 0;
+        } else {
+            const double h = floor(U * 0x1p-32);
+            lo[j] = (uint32_t)fma(h, -0x1p32, U);
+            hi[j] = (uint32_t)h;
         }
     }
-    int sh[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        double mm = m[i];
-        for (int off = 32; off >= 1; off >>= 1) mm = fmax(mm, __shfl_xor(mm, off, 64));
-        int e = 0;
-        if (mm > 0.0) frexp(mm, &e);
-        if (lane == 0) rexp[rb * 16 + 4 * w + i] = e;
-        sh[i] = 8 * nslice - 2 - e;
-    }
-    const size_t plane16 = (size_t)rows_p * K_p / 16;  // plane size in uint4
-    const int64_t nk = K_p >> 6;
-    uint32_t *st32 = (uint32_t *)stage;
-    // this lane's four elements in the chunk: K step lane / 16, bytes
-    // 4 (lane % 16) .. +3 of it, i.e. kq = (lane % 16) / 4, byte 4 (lane % 4)
-    const int ks = lane >> 4, kq = (lane & 15) >> 2, byte = 4 * (lane & 3);
-    for (int64_t c0 = 0; c0 < K_p; c0 += 256) {
-        const int64_t c = c0 + 4 * lane;
-        const int64_t o0 = c < K ? v.cofs[c] : 0, o2 = c + 2 < K ? v.cofs[c + 2] : 0;
+    for (int p = 0; p < NS; p++) {
+        const int b = NS - 1 - p;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            d2 a = {0.0, 0.0}, b = {0.0, 0.0};
-            if (real[i] && c < K) {
-                a = *(const d2 *)(xr[i] + o0);
-                if (c + 2 < K) b = *(const d2 *)(xr[i] + o2);
-            }
-            const double xs[4] = {a.x, a.y, b.x, b.y};
-            uint32_t wd[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 4; k++) {
+            uint32_t w = 0;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                int q[6] = {0, 0, 0, 0, 0, 0};
-                i8_digits(xs[j], sh[i], nslice, q);
-#pragma unroll
-                for (int s = 0; s < 6; s++) wd[s] |= (uint32_t)(q[s] & 255) << (8 * j);
+                const uint32_t u = b < 4 ? lo[4 * k + j] : hi[4 * k + j];
+                w |= ((u >> (8 * (b & 3))) & 255u) << (8 * j);
             }
-            const int r = 4 * w + i;
-            const int o = ((ks * 64 + r + 16 * kq) * 16 + byte) >> 2;
-#pragma unroll
-            for (int s = 0; s < 6; s++)
-                if (s < nslice) st32[s * 1024 + o] = wd[s];
+            wd[p][k] = w ^ 0x80808080u;
         }
-        __syncthreads();
-        const int64_t kt0 = c0 >> 6;
-        if (kt0 + (tid >> 6) < nk) {  // K_p may end halfway through a chunk
-            uint4 *out = (uint4 *)dst + (rb * nk + kt0) * 64 + tid;
-#pragma unroll
-            for (int s = 0; s < 6; s++)
-                if (s < nslice) out[s * plane16] = stage[s * 256 + tid];
-        }
-        __syncthreads();
     }
 }
 
-// INT8 split of a view without a block table whose rows are contiguous in
-// runs of 16 (rofs[r] = rofs[r & ~15] + r % 16: a row-major operand read
-// transposed, as the ring's integrals and the ladder's amplitudes are).
-// One workgroup of 16 waves per block of 16 rows, wave w on the w-th
-// sixteenth of the K steps; lane l = r + 16 kq on row r.  Each load
-// instruction reads 16 consecutive rows of 4 columns (whole cache lines);
-// the row maxima of the 16 waves meet in LDS; then lane l converts row r's
-// elements 16 kq .. 16 kq + 15 of each of its K steps -- its own lane of
-// the fragment -- so each plane gets one 16-byte store per lane and whole
-// 1 KiB fragments.  Same digits as split_kernel.
-__global__ __launch_bounds__(1024) void split_i8_cols_kernel(const double *__restrict__ src, ViewArgs v,
-                                                            int64_t rows, int64_t K, int64_t rows_p,
-                                                            int64_t K_p, int nslice,
-                                                            int8_t *__restrict__ dst,
-                                                            int32_t *__restrict__ rexp)
+__global__ __launch_bounds__(256) void i8_exp_init(int32_t *__restrict__ rexp, int64_t rows_p)
 {
-    __shared__ double smax[16][16];  // [wave][row of the block]
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
-    const int64_t rb = blockIdx.x;
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < rows_p; r += (int64_t)gridDim.x * 256)
+        rexp[r] = INT_MIN;
+}
+
+template <bool PAIRS>
+__global__ __launch_bounds__(256) void i8_exp_kernel(const double *__restrict__ src, ViewArgs v,
+                                                     int64_t rows, int64_t K, int64_t nrb,
+                                                     int32_t *__restrict__ rexp)
+{
+    constexpr int KC = 1024;
+    const int t = threadIdx.x;
+    const int64_t rb = (int64_t)blockIdx.x % nrb, ch = (int64_t)blockIdx.x / nrb;  // row block fastest
+    // PAIRS: lanes along a row's columns; else lanes along the 16 rows of a column
+    const int r = PAIRS ? (t >> 4) : (t & 15), cg = PAIRS ? (t & 15) : (t >> 4);
     const int64_t row = rb * 16 + r;
-    const bool real = row < rows;
-    const double *x = real ? src + v.rofs[rb * 16] + r : src;  // rofs[row] = rofs[rb*16] + r
-    const int64_t nk = K_p >> 6, per = (nk + 15) / 16;
-    const int64_t kt0 = min(nk, w * per), kt1 = min(nk, kt0 + per);
-    const int64_t cend = min(K, kt1 * 64);
-    double m0 = 0.0, m1 = 0.0;
-    for (int64_t c = kt0 * 64 + kq; real && c < cend; c += 8) {
-        m0 = fmax(m0, fabs(x[v.cofs[c]]));
-        if (c + 4 < cend) m1 = fmax(m1, fabs(x[v.cofs[c + 4]]));
+    const int64_t c0 = ch * KC;
+    const int64_t cend = min(K, c0 + KC);
+    double m = 0.0;
+    if (row < rows) {
+        const double *x = src + v.rofs[row];
+        if (PAIRS) {  // 16-byte loads of (c, c + 1); K is even
+            for (int64_t cb = c0 + 2 * cg; cb < cend; cb += 256) {
+                int64_t o[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) o[u] = cb + 32 * u < cend ? v.cofs[cb + 32 * u] : -1;
+                d2 a[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) a[u] = o[u] >= 0 ? ld_nt2(x + o[u]) : d2{0.0, 0.0};
+#pragma unroll
+                for (int u = 0; u < 8; u++) m = fmax(m, fmax(fabs(a[u].x), fabs(a[u].y)));
+            }
+        } else {
+            for (int64_t cb = c0 + cg; cb < cend; cb += 128) {
+                int64_t o[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) o[u] = cb + 16 * u < cend ? v.cofs[cb + 16 * u] : -1;
+                double a[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) a[u] = o[u] >= 0 ? ld_nt(x + o[u]) : 0.0;
+#pragma unroll
+                for (int u = 0; u < 8; u++) m = fmax(m, fabs(a[u]));
+            }
+        }
     }
-    double m = fmax(m0, m1);
-    m = fmax(m, __shfl_xor(m, 16, 64));
-    m = fmax(m, __shfl_xor(m, 32, 64));
-    if (kq == 0) smax[w][r] = m;
+    // reduce over the threads of one row: 16 lanes (RUN: every 16th lane)
+    if (PAIRS) {
+        for (int off = 8; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    } else {
+        m = fmax(m, __shfl_xor(m, 16, 64));
+        m = fmax(m, __shfl_xor(m, 32, 64));
+    }
+    __shared__ double sm[4][16];
+    if (PAIRS) {
+        if (cg == 0) sm[0][r] = m;
+    } else {
+        if ((t & 63) < 16) sm[t >> 6][r] = m;
+    }
     __syncthreads();
-    m = 0.0;
+    if (t < 16) {
+        const int64_t rr = rb * 16 + t;
+        const double mm = PAIRS ? sm[0][t] : fmax(fmax(sm[0][t], sm[1][t]), fmax(sm[2][t], sm[3][t]));
+        if (rr < rows && mm > 0.0) {
+            int e;
+            frexp(mm, &e);
+            atomicMax(&rexp[rr], e);
+        }
+    }
+}
+
+template <bool PAIRS, int NS>
+__global__ __launch_bounds__(256) void i8_digit_kernel(const double *__restrict__ src, ViewArgs v,
+                                                       int64_t rows, int64_t K, int64_t rows_p,
+                                                       int64_t K_p, int64_t nrb,
+                                                       int8_t *__restrict__ dst,
+                                                       int32_t *__restrict__ rexp)
+{
+    constexpr int S = 66;  // doubles per staged row: 16-byte aligned, padded
+    __shared__ __attribute__((aligned(16))) double tile[PAIRS ? 4 : 1][PAIRS ? 16 * S : 2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
+    const int64_t rb = (int64_t)blockIdx.x % nrb, ch = (int64_t)blockIdx.x / nrb, nk = K_p >> 6;
+    const int64_t kt = ch * 4 + w;
+    if (kt >= nk) return;  // whole waves leave: no workgroup barrier below
+    const int64_t row = rb * 16 + r;
+    int e = rexp[row];
+    if (e == INT_MIN) {  // all-zero row (or padding): exponent 0
+        e = 0;
+        if (ch == 0 && w == 0 && kq == 0) rexp[row] = 0;
+    }
+    const int sh = 8 * NS - 2 - e;
+    double xs[16];
+    if constexpr (PAIRS) {
+        double *tw = tile[w];
+        const int cp = lane & 31, rh = lane >> 5;
+        const int64_t col = kt * 64 + 2 * cp;
+        const int64_t co = col < K ? v.cofs[col] : 0;
+        d2 a[8];
 #pragma unroll
-    for (int u = 0; u < 16; u++) m = fmax(m, smax[u][r]);
-    int e = 0;
-    if (m > 0.0) frexp(m, &e);
-    if (w == 0 && kq == 0) rexp[row] = e;
-    const size_t plane16 = (size_t)rows_p * K_p / 16;
-    const int sh = 8 * nslice - 2 - e;
-    uint4 *out = (uint4 *)dst + rb * nk * 64 + lane;
-    for (int64_t kt = kt0; kt < kt1; kt++) {
-        const int64_t c = kt * 64 + 16 * kq;
-        double xs[16];
-#pragma unroll
-        for (int j = 0; j < 16; j++) xs[j] = (real && c + j < K) ? x[v.cofs[c + j]] : 0.0;
-        uint32_t wd[6][4] = {};
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            int q[6] = {0, 0, 0, 0, 0, 0};
-            i8_digits(xs[j], sh, nslice, q);
-#pragma unroll
-            for (int s = 0; s < 6; s++) wd[s][j >> 2] |= (uint32_t)(q[s] & 255) << (8 * (j & 3));
+        for (int u = 0; u < 8; u++) {
+            const int64_t rr = rb * 16 + 2 * u + rh;
+            a[u] = (rr < rows && col < K) ? ld_nt2(src + v.rofs[rr] + co) : d2{0.0, 0.0};
         }
 #pragma unroll
-        for (int s = 0; s < 6; s++)
-            if (s < nslice) out[s * plane16 + kt * 64] = make_uint4(wd[s][0], wd[s][1], wd[s][2], wd[s][3]);
+        for (int u = 0; u < 8; u++) *(d2 *)(tw + (2 * u + rh) * S + 2 * cp) = a[u];
+        // the tile is this wave's own: LDS accesses of one wave complete in
+        // order, so only the compiler must keep the reads below the writes
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+            const d2 b = *(const d2 *)(tw + r * S + 16 * kq + j);
+            xs[j] = b.x;
+            xs[j + 1] = b.y;
+        }
+    } else {
+        const int64_t c = kt * 64 + 16 * kq;
+        if (row < rows) {
+            const double *x = src + v.rofs[row];
+#pragma unroll
+            for (int j = 0; j < 16; j++) xs[j] = c + j < K ? ld_nt(x + v.cofs[c + j]) : 0.0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; j++) xs[j] = 0.0;
+        }
     }
+    uint32_t wd[6][4];
+    i8_digit_words<NS>(xs, sh, wd);
+    const size_t plane16 = (size_t)rows_p * K_p / 16;
+    uint4 *out = (uint4 *)dst + (rb * nk + kt) * 64 + lane;
+#pragma unroll
+    for (int p = 0; p < NS; p++) st_nt(out + p * plane16, wd[p][0], wd[p][1], wd[p][2], wd[p][3]);
 }
 
 // Wide operands (few rows, long K: the T1 statements' A operands are a few
-// hundred rows by millions of K) leave the row-parallel kernels above with
-// a few hundred waves.  There the split runs in two K-parallel launches:
-// one block per row reduces the row maximum to its exponent, then every
-// thread writes one 16-byte chunk of each slice plane (16 INT8, 8 BF16 /
-// FP16 or, row-major, 4 FP32 elements) -- the same values, bit for bit, as
-// the row-parallel kernels.
+// hundred rows by millions of K) leave the row-parallel split_kernel with a
+// few hundred waves.  There the float splits (and INT8 splits of views
+// with a block table) run in two K-parallel launches: one block per row
+// reduces the row maximum to its exponent, then every thread writes one
+// 16-byte chunk of each slice plane (16 INT8, 8 BF16 / FP16 or, row-major,
+// 4 FP32 elements) -- the same values, bit for bit, as split_kernel.
 template <int MODE>
 __global__ __launch_bounds__(512) void row_exp_kernel(const double *__restrict__ src, ViewArgs v,
                                                       int64_t rows, int64_t K,
@@ -1214,11 +1258,39 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
     const int64_t rp = round_up(rows, 128), kp = round_up(K, 128);
     const unsigned grid = (unsigned)((rp + 3) / 4);
     hipStream_t s = as_stream(stream);
-    // Waves the row-parallel kernel would run: the INT8 column-pair kernel 4
-    // per 16 rows, the row-run kernel 16 per 16 rows, the generic one per row.
-    const bool i8rows = prec == DTCF_PREC_I8 && !v->blk && v->colpairs &&
-                        (reinterpret_cast<uintptr_t>(src) & 15) == 0;
-    const int64_t row_waves = i8rows ? rp / 4 : rp;
+    // INT8 table views (no block table) split in two K-parallel passes at
+    // any shape: exponents, then digits.
+    if (prec == DTCF_PREC_I8 && !v->blk) {
+        const bool pairs = v->colpairs && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+        // one-dimensional grids, row blocks fastest
+        const int64_t nrb = rp / 16, ne = nrb * ((K + 1023) / 1024), nd = nrb * ((kp / 64 + 3) / 4);
+        DTCF_REQUIRE(ne < (int64_t(1) << 31) && nd < (int64_t(1) << 31), "split: operand too large");
+        const dim3 ge((unsigned)ne), gd((unsigned)nd);
+        int8_t *d8 = (int8_t *)dst;
+        hipLaunchKernelGGL(i8_exp_init, dim3((unsigned)std::min<int64_t>((rp + 255) / 256, 4096)),
+                           dim3(256), 0, s, rexp, rp);
+        if (pairs)
+            hipLaunchKernelGGL(i8_exp_kernel<true>, ge, dim3(256), 0, s, src, a, rows, K, nrb, rexp);
+        else
+            hipLaunchKernelGGL(i8_exp_kernel<false>, ge, dim3(256), 0, s, src, a, rows, K, nrb, rexp);
+#define DTCF_DIGITS(NS)                                                                                \
+    if (pairs) hipLaunchKernelGGL((i8_digit_kernel<true, NS>), gd, dim3(256), 0, s, src, a, rows, K, rp, \
+                                  kp, nrb, d8, rexp);                                                  \
+    else hipLaunchKernelGGL((i8_digit_kernel<false, NS>), gd, dim3(256), 0, s, src, a, rows, K, rp, kp, \
+                            nrb, d8, rexp);
+        switch (nslice) {
+        case 1: DTCF_DIGITS(1) break;
+        case 2: DTCF_DIGITS(2) break;
+        case 3: DTCF_DIGITS(3) break;
+        case 4: DTCF_DIGITS(4) break;
+        case 5: DTCF_DIGITS(5) break;
+        default: DTCF_DIGITS(6) break;
+        }
+#undef DTCF_DIGITS
+        DTCF_HIP(hipGetLastError());
+        return DTCF_OK;
+    }
+    const int64_t row_waves = rp;  // the row-parallel kernel runs a wave per row
     if (row_waves < 2048 && kp >= 8 * rp) {  // wide: K-parallel launches
         const int ns = prec == DTCF_PREC_I8 ? nslice : prec == DTCF_PREC_F32 ? 1 : 3;
         const int64_t esz = prec == DTCF_PREC_F32 ? 4 : prec == DTCF_PREC_I8 ? 1 : 2;
@@ -1263,15 +1335,8 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
                            K, rp, kp, 3, dst, rexp);
         break;
     default:
-        if (!v->blk && v->colpairs && (reinterpret_cast<uintptr_t>(src) & 15) == 0)
-            hipLaunchKernelGGL(split_i8_rows_kernel, dim3((unsigned)(rp / 16)), dim3(256), 0, s, src, a,
-                               rows, K, rp, kp, nslice, (int8_t *)dst, rexp);
-        else if (v->rowrun16)
-            hipLaunchKernelGGL(split_i8_cols_kernel, dim3((unsigned)(rp / 16)), dim3(1024), 0, s, src, a,
-                               rows, K, rp, kp, nslice, (int8_t *)dst, rexp);
-        else
-            hipLaunchKernelGGL(split_kernel<DTCF_PREC_I8>, dim3(grid), dim3(256), 0, s, src, a, rows,
-                               K, rp, kp, nslice, dst, rexp);
+        hipLaunchKernelGGL(split_kernel<DTCF_PREC_I8>, dim3(grid), dim3(256), 0, s, src, a, rows, K,
+                           rp, kp, nslice, dst, rexp);
         break;
     }
     DTCF_HIP(hipGetLastError());

@@ -509,12 +509,17 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
     // thinner contractions (the T1 terms, n = o) stay FP64.
     s.mixed = p->prec != 0 &&
               (double)s.m * (double)s.n >= p->mixed_min * (double)(s.m + s.n);
-    // FP64: an operand with neither direction contiguous is permuted into a
-    // dense [rows][K] copy first (one pass at HBM speed) unless it is small.
-    // Mixed precision re-lays operands out in its split pass anyway.
+    // An operand with neither direction contiguous is permuted into a dense
+    // [rows][K] copy first (one pass at HBM speed) unless it is small: the
+    // FP64 GEMM reads it from there, and so does the mixed-precision split
+    // pass, whose kernels read rows or column pairs contiguously and would
+    // otherwise gather single elements.  Resident inputs are split once, at
+    // bind, from the array itself.
+    const int garr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};  // GEMM operand q's array
     int64_t scr = 0;
     std::vector<int64_t> dense_r[2], dense_k[2];
-    for (int q = 0; q < 2 && !s.mixed; q++) {
+    for (int q = 0; q < 2; q++) {
+        if (s.mixed && p->arrs[garr[q]].role == DTCF_ARR_INPUT && p->arrs[garr[q]].resident) continue;
         const auto &R = q == 0 ? Ar : Br;
         const auto &Kk = q == 0 ? Ak : Bk;
         const int64_t rows = (int64_t)R.size(), K = (int64_t)Kk.size();
@@ -566,11 +571,9 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
         s.sa_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.m, s.k));
         s.sb_bytes = granule(dtcf_split_bytes(p->prec, p->nslice, s.n, s.k));
         DTCF_REQUIRE(s.sa_bytes > 0 && s.sb_bytes > 0, "program: bad mixed-precision mode");
-        // GEMM operand 0 is array b when swapped, a otherwise
-        const int arr[2] = {s.swap ? s.b : s.a, s.swap ? s.a : s.b};
         for (int q = 0; q < 2; q++) {
-            s.res[q] = p->arrs[arr[q]].role == DTCF_ARR_INPUT && p->arrs[arr[q]].resident;
-            if (p->arrs[arr[q]].role != DTCF_ARR_INPUT) continue;
+            s.res[q] = p->arrs[garr[q]].role == DTCF_ARR_INPUT && p->arrs[garr[q]].resident;
+            if (p->arrs[garr[q]].role != DTCF_ARR_INPUT) continue;
             const auto &R = q == 0 ? Ar : Br;
             const auto &Kk = q == 0 ? Ak : Bk;
             auto &key = s.vkey[q];
@@ -580,7 +583,7 @@ int plan_contraction(dtcf_program p, PStmt &s, const std::vector<std::string> &c
             key.insert(key.end(), R.begin(), R.end());
             key.insert(key.end(), Kk.begin(), Kk.end());
         }
-        int64_t off = 0;
+        int64_t off = scr;  // after the dense copies, if any
         if (!s.res[0]) { s.a_off = off; off += s.sa_bytes; }
         if (!s.res[1]) { s.b_off = off; off += s.sb_bytes; }
         if (!s.res[0]) { s.ea_off = off; off += granule((s.m + 127) / 128 * 128 * 4); }
@@ -678,9 +681,16 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
     }
     char *sa = sl[0], *sb = sl[1];
     int32_t *ea = ex[0], *eb = ex[1];
+    const double *op[2] = {A, B};
     int st = DTCF_OK;
-    if (split[0]) st = dtcf_split_f64(p->prec, p->nslice, A, s.av, sa, ea, stream);
-    if (st == DTCF_OK && split[1]) st = dtcf_split_f64(p->prec, p->nslice, B, s.bv, sb, eb, stream);
+    for (int q = 0; q < 2 && st == DTCF_OK; q++) {
+        if (!split[q] || !s.pack[q]) continue;
+        double *dst = reinterpret_cast<double *>(r->base + s.scratch_off + s.poff[q]);
+        st = dtcf_permute_copy_f64(op[q], dst, &s.pdesc[q], 1, nullptr, 0, nullptr, 0, 1.0, 0.0, stream);
+        op[q] = dst;
+    }
+    if (st == DTCF_OK && split[0]) st = dtcf_split_f64(p->prec, p->nslice, op[0], s.av, sa, ea, stream);
+    if (st == DTCF_OK && split[1]) st = dtcf_split_f64(p->prec, p->nslice, op[1], s.bv, sb, eb, stream);
     if (st == DTCF_OK)
         st = dtcf_gemm_split_f64(p->prec, p->nslice, s.m, s.n, s.k, s.alpha, sa, ea, sb, eb,
                                  p->max_level, c, s.cv, s.ldc, stream);

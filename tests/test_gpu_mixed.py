@@ -267,24 +267,43 @@ def test_float_split_bitwise_vs_host(dev, prec, rows, K):
         r = r - qd
 
 
-@pytest.mark.parametrize("ns", [4, 5, 6])
-@pytest.mark.parametrize("rows,K", [(200, 522), (128, 256), (17, 66), (40, 20000)])
+@pytest.mark.parametrize("ns", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("rows,K", [(200, 522), (128, 256), (17, 66), (40, 20000), (600, 3000)])
 def test_int8_split_layout_and_digits(dev, ns, rows, K):
-    """One matrix stored four ways -- row-major (row kernel), row-major with
-    its column pairs permuted (row kernel through the column table),
-    column-major (column kernel), and row-major behind a one-block table
-    (generic kernel) -- gives identical planes; decoded from the
-    fragment-major layout, the digits lie in [-128, 127] and reconstruct
-    X = rint(x 2^(8ns-2-e)) with e the exponent of the row max (frexp).
-    40 x 20000 is wide: all four take the K-parallel launches."""
+    """One matrix stored five ways -- row-major (two-pass column-pair
+    kernels), row-major with its column pairs permuted (the same through the
+    column table), column-major (two-pass kernels, 16-row runs), row-major
+    with odd row offsets (two-pass kernels, gathered), and as 2 x 2 blocks
+    in a non-separable block order (generic kernel; rows even) -- gives
+    identical planes; decoded from the fragment-major layout, the digits
+    lie in [-128, 127] and reconstruct X = rint(x 2^(8ns-2-e)) with e the
+    exponent of the row max (frexp).  40 x 20000 and 600 x 3000 merge row
+    exponents from several 1024-column chunks; 40 x 20000 takes the wide
+    launches for the block-table view."""
     rng = np.random.default_rng(rows * 7 + K + ns)
     A = rng.standard_normal((rows, K)) * np.exp2(rng.integers(-20, 20, size=(rows, 1)))
     A[3 % rows] = 0.0  # an all-zero row: e = 0, all digits 0
     Ad = torch.from_numpy(A).to(dev)
     fast = mixed.Split(mixed.I8, Ad, mixed.dense_view(rows, K), ns)
-    gen_view = ops.View(rows, K, np.arange(rows) * K, np.arange(K), blk=np.zeros(1, np.int64),
-                        rpb=rows, cpb=K)
-    gen = mixed.Split(mixed.I8, Ad, gen_view, ns)
+    others = []
+    if rows % 2 == 0:
+        # blocks (0,0) (0,1) (1,0) (1,1) stored at slots 0 1 3 2: blk(1,1) is
+        # not blk(1,0) + blk(0,1) - blk(0,0), so the table stays
+        h, w = rows // 2, K // 2
+        slot = np.array([0, 1, 3, 2], np.int64)
+        Ab = np.empty(rows * K)
+        for R in range(2):
+            for Cb in range(2):
+                o = slot[2 * R + Cb] * h * w
+                Ab[o:o + h * w] = A[R * h:(R + 1) * h, Cb * w:(Cb + 1) * w].ravel()
+        gen_view = ops.View(rows, K, (np.arange(rows) % h) * w, np.arange(K) % w, blk=slot * h * w,
+                            rpb=h, cpb=w)
+        others.append(mixed.Split(mixed.I8, torch.from_numpy(Ab).to(dev), gen_view, ns))
+    # rows at odd offsets (stride K + 1 after one leading element): no pairs, no runs
+    Ao = np.zeros(1 + rows * (K + 1))
+    Ao[1:].reshape(rows, K + 1)[:, :K] = A
+    others.append(mixed.Split(mixed.I8, torch.from_numpy(Ao).to(dev),
+                              ops.View(rows, K, 1 + np.arange(rows) * (K + 1), np.arange(K)), ns))
     # column pairs stored in reverse order: cofs[c] = (K/2 - 1 - c//2) * 2 + c % 2
     cp = np.arange(K)
     cofs = (K // 2 - 1 - cp // 2) * 2 + cp % 2
@@ -295,7 +314,7 @@ def test_int8_split_layout_and_digits(dev, ns, rows, K):
     At = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
     colmaj = mixed.Split(mixed.I8, At, ops.View(rows, K, np.arange(rows), np.arange(K) * rows), ns)
     torch.cuda.synchronize()
-    for other in (gen, perm, colmaj):
+    for other in others + [perm, colmaj]:
         assert torch.equal(fast.buf, other.buf) and torch.equal(fast.rexp, other.rexp)
     rp, kp = (rows + 127) // 128 * 128, (K + 127) // 128 * 128
     planes = fast.buf.cpu().numpy().view(np.int8).reshape(ns, rp // 16, kp // 64, 4, 16, 16)
