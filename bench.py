@@ -298,13 +298,19 @@ def leg_precision(log):
 
 def leg_blocksparse(dev, log, steps=5):
     """Block-sparse D2h terms (o=64, v=256, 8 irreps): one grouped persistent
-    launch per term over its symmetry-allowed block pairs."""
+    launch per term over its symmetry-allowed block pairs; then the
+    dense-by-irrep form in FP64 against INT8 slice modes (time, speedup and
+    the error of C against FP64, per term)."""
     from dtcf_amd import blocksparse as bsp
     out = bsp.bench_terms(dev, steps=steps)
     for k, v in out.items():
         log(f"[blocksparse] {k}: {v}")
     for v in out.values():
         v["frac_fp64_peak"] = round(v["tflops"] / FP64_PEAK_TF, 4)
+    mx = bsp.bench_mixed_terms(dev, steps=steps, modes=("int8x4/L5", "int8x3/L4"))
+    for k, v in mx.items():
+        log(f"[blocksparse mixed] {k}: {v}")
+    out["mixed_dense_by_irrep"] = mx
     return out
 
 

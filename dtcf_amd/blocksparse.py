@@ -330,19 +330,74 @@ class IrrepDense:
                 best = (ms, cfg)
         self.gemm_cfg = best[1]
 
+    def set_precision(self, prec=0, nslice=4, max_level=0):
+        """prec 0: FP64 (one grouped launch).  A mixed mode (mixed.F32,
+        BF16X3, F16X3, I8 with nslice digits and products up to max_level):
+        every A_h and B_h is split into slices (one two-pass split per
+        matrix, dense row-major views) and each C_h is the sum of the slice
+        products (dtcf_gemm_split_f64, FP64 recombination).  The slice
+        buffers are allocated here, once."""
+        from . import mixed, ops
+        self.prec, self.max_level = prec, max_level
+        self._mx = None
+        if prec == 0:
+            return
+        ns = nslice if prec == mixed.I8 else (1 if prec == mixed.F32 else 3)
+        self.nslice = ns
+        dev = self.dA.device
+        mx = []
+        for m, n, k, ao, bo, co in self.gemms:
+            ent = {}
+            for tag, rows, base in (("a", m, ao), ("b", n, bo)):
+                view = ops.View(rows, k, base + np.arange(rows, dtype=np.int64) * k,
+                                np.arange(k, dtype=np.int64))
+                nb = lib().dtcf_split_bytes(prec, ns, rows, k)
+                buf = torch.empty(nb, dtype=torch.uint8, device=dev)
+                rexp = torch.zeros((rows + 127) // 128 * 128, dtype=torch.int32, device=dev)
+                ent[tag] = (view, buf, rexp)
+            mx.append(ent)
+        self._mx = mx
+
+    def _split(self, tag, src):
+        for ent in self._mx:
+            view, buf, rexp = ent[tag]
+            check(lib().dtcf_split_f64(self.prec, self.nslice, C.c_void_p(src.data_ptr()), view.h,
+                                       C.c_void_p(buf.data_ptr()), C.c_void_p(rexp.data_ptr()), _s()))
+
+    def _mixed_gemms(self):
+        self.dC.zero_()  # the slice products accumulate
+        c0 = self.dC.data_ptr()
+        for (m, n, k, ao, bo, co), ent in zip(self.gemms, self._mx):
+            _, ab, ae = ent["a"]
+            _, bb, be = ent["b"]
+            check(lib().dtcf_gemm_split_f64(self.prec, self.nslice, m, n, k, 1.0,
+                                            C.c_void_p(ab.data_ptr()), C.c_void_p(ae.data_ptr()),
+                                            C.c_void_p(bb.data_ptr()), C.c_void_p(be.data_ptr()),
+                                            int(self.max_level), C.c_void_p(c0 + 8 * co), None, n,
+                                            _s()))
+
     def run(self, alpha, A, B, Cb, a_resident=False):
         """Gathers A and B into the per-irrep matrices, computes every
         C_h = A_h B_h^T in one grouped launch (overwriting C_h), then adds
         alpha * C_h into C's blocks, all on the caller's stream.
         a_resident: the caller promises A is the buffer of the previous run,
-        unchanged since (fixed integrals), so its gathered form is reused."""
-        if not (a_resident and getattr(self, "_a_src", None) == A.data_ptr()):
+        unchanged since (fixed integrals), so its gathered form (and, in a
+        mixed mode, its slices) is reused."""
+        mixed_mode = getattr(self, "_mx", None) is not None
+        key = (A.data_ptr(), getattr(self, "prec", 0), getattr(self, "nslice", 0))
+        if not (a_resident and getattr(self, "_a_src", None) == key):
             self.mA.gather(A, self.dA)
-            self._a_src = A.data_ptr()
+            if mixed_mode:
+                self._split("a", self.dA)
+            self._a_src = key
         self.mB.gather(B, self.dB)
-        if getattr(self, "gemm_cfg", None) is None:
-            self._tune()
-        self._gemms(self.gemm_cfg)
+        if mixed_mode:
+            self._split("b", self.dB)
+            self._mixed_gemms()
+        else:
+            if getattr(self, "gemm_cfg", None) is None:
+                self._tune()
+            self._gemms(self.gemm_cfg)
         self.mC.scatter(alpha, self.dC, Cb)
 
 
@@ -431,6 +486,15 @@ class BlockTerm:
         self.flops = fl.value
         self.nwork = nw.value
         assert spans[0] <= self.A.size and spans[1] <= self.B.size and spans[2] <= self.C.size
+
+    def set_precision(self, prec=0, nslice=4, max_level=0):
+        """Mixed-precision contraction (see IrrepDense.set_precision); only
+        the dense-by-irrep configuration (4) has one."""
+        if prec != 0 and self.cfg != 4:
+            raise ValueError("block-sparse mixed precision runs in the dense-by-irrep "
+                             "configuration (cfg 4): call replan(4) first")
+        if self._dense is not None:
+            self._dense.set_precision(prec, nslice, max_level)
 
     def replan(self, cfg):
         if self.h:
@@ -580,12 +644,90 @@ def bench_terms(dev="cuda", o=64, v=256, seed=5, steps=5):
     return out
 
 
+# mixed modes of the dense-by-irrep configuration: (precision, digits, level)
+MIXED_MODES = {"int8x5/L6": (4, 5, 6), "int8x4/L5": (4, 4, 5), "int8x3/L4": (4, 3, 4)}
+
+
+def bench_mixed_terms(dev="cuda", o=64, v=256, seed=5, steps=5, modes=("int8x4/L5", "int8x3/L4")):
+    """The D2h ring and ladder terms at o, v in the dense-by-irrep
+    configuration: FP64 against each mixed mode.  Per term and mode: ms per
+    run (gathers, splits, slice products and scatter), ms with A (the
+    integrals) resident as gathered slices, speedup over the FP64 form of
+    the same configuration and over the autotuned FP64 configuration, and
+    the error of C against FP64 (largest element and Frobenius norm, both
+    relative to C's largest element / norm)."""
+    from . import mixed
+    sp = Space.seeded(o=o, v=v, seed=seed)
+    out = {}
+
+    def timed(term, Cb, res, n):
+        term.run(1.0, Cb=Cb, a_resident=res)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            term.run(1.0, Cb=Cb, a_resident=res)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    for name, ((ck, ci), (ak, ai), (bk, bi)) in BENCH_TERMS.items():
+        A = BSTensor(sp, ak, dev)
+        A.data.copy_(mixed.fill_normal(A.size, 1, 0.05))
+        B = BSTensor(sp, bk, dev)
+        B.data.copy_(mixed.fill_normal(B.size, 2, 0.05))
+        Ct = BSTensor(sp, ck, dev)
+        term = BlockTerm(Ct, ci, A, ai, B, bi)
+        best_cfg, times = autotune(term)
+        ms_best = times[best_cfg]
+        term.replan(4)
+        scratch = torch.zeros_like(Ct.data)
+        ms64 = timed(term, scratch, False, steps)
+        ms64r = timed(term, scratch, True, steps)
+        Ct.data.zero_()
+        term.run(1.0)
+        ref = Ct.data.clone()
+        rmax = float(ref.abs().max())
+        rnorm = float(torch.linalg.vector_norm(ref))
+        row = {"block_pairs": term.npairs, "flops": term.flops,
+               "fp64_best": {"tile_config": CONFIGS[best_cfg], "ms": round(ms_best, 4),
+                             "tflops": round(term.flops / ms_best / 1e9, 3)},
+               "fp64_dense_by_irrep": {"ms": round(ms64, 4), "ms_a_resident": round(ms64r, 4)}}
+        for mode in modes:
+            prec, ns, lev = MIXED_MODES[mode]
+            term.set_precision(prec, ns, lev)
+            ms = timed(term, scratch, False, steps)
+            msr = timed(term, scratch, True, steps)
+            Ct.data.zero_()
+            term.run(1.0)
+            d = Ct.data - ref
+            row[mode] = {"ms": round(ms, 4), "ms_a_resident": round(msr, 4),
+                         "tflops_equiv": round(term.flops / ms / 1e9, 3),
+                         "speedup_vs_fp64_dense_by_irrep": round(ms64 / ms, 3),
+                         "speedup_a_resident": round(ms64r / msr, 3),
+                         "speedup_vs_fp64_best": round(ms_best / ms, 3),
+                         "rel_err_max": float(d.abs().max()) / rmax,
+                         "rel_err_frobenius": float(torch.linalg.vector_norm(d)) / rnorm}
+            term.set_precision(0)
+        out[name] = row
+        del A, B, Ct, term, scratch, ref
+        torch.cuda.empty_cache()
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="D2h block-sparse ring and ladder terms, grouped launch")
     ap.add_argument("--o", type=int, default=64)
     ap.add_argument("--v", type=int, default=256)
     ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--mixed", nargs="*", default=None, metavar="MODE",
+                    help=f"mixed-precision comparison in the dense-by-irrep form ({', '.join(MIXED_MODES)})")
     args = ap.parse_args()
     check(lib().dtcf_init(0))
-    print(json.dumps(bench_terms("cuda", args.o, args.v, steps=args.steps), indent=1), flush=True)
+    if args.mixed is not None:
+        res = bench_mixed_terms("cuda", args.o, args.v, steps=args.steps,
+                                modes=tuple(args.mixed) or ("int8x4/L5", "int8x3/L4"))
+    else:
+        res = bench_terms("cuda", args.o, args.v, steps=args.steps)
+    print(json.dumps(res, indent=1), flush=True)

@@ -8,6 +8,7 @@ import torch
 
 import oracle
 from dtcf_amd import blocksparse as bsp
+from dtcf_amd import mixed
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -110,3 +111,46 @@ def test_irrep_dense_tiles_and_resident_a(dev, term, gcfg):
     Ct.data.copy_(torch.from_numpy(C0))
     t.run(0.5, a_resident=True)
     assert np.array_equal(Ct.data.cpu().numpy(), C1)
+
+
+# (precision, digits, level) -> bound on the error of C relative to its largest element
+MIXED = [((mixed.I8, 5, 6), 5e-10), ((mixed.I8, 4, 5), 1e-7), ((mixed.I8, 3, 4), 2e-5),
+         ((mixed.BF16X3, 3, 0), 5e-6), ((mixed.F16X3, 3, 0), 5e-6), ((mixed.F32, 1, 0), 5e-6)]
+
+
+@pytest.mark.parametrize("mode,bound", MIXED, ids=lambda x: str(x))
+@pytest.mark.parametrize("term", TERMS[:2], ids=["ring", "ladder"])
+def test_mixed_dense_by_irrep_vs_fp64(dev, term, mode, bound):
+    """The dense-by-irrep form in a mixed mode against its FP64 form on the
+    same block-sparse operands; a second run with A resident (slices kept)
+    gives the same C bit for bit, and switching back to FP64 gives the FP64
+    C bit for bit."""
+    ck, ci, ak, ai, bk, bi = term
+    sp = bsp.Space.seeded(o=16, v=40, seed=3)
+    A = bsp.BSTensor(sp, ak, dev).fill_normal(1)
+    B = bsp.BSTensor(sp, bk, dev).fill_normal(2)
+    C64 = bsp.BSTensor(sp, ck, dev)
+    t = bsp.BlockTerm(C64, ci, A, ai, B, bi, cfg=4)
+    t.run(1.0)
+    Cm = bsp.BSTensor(sp, ck, dev)
+    t.set_precision(*mode)
+    t.run(1.0, Cb=Cm.data)
+    ref, got = C64.data.cpu().numpy(), Cm.data.cpu().numpy()
+    err = relerr(got, ref)
+    print(term[1], mode, "rel err", err, flush=True)
+    assert err <= bound
+    again = torch.zeros_like(Cm.data)
+    t.run(1.0, Cb=again, a_resident=True)
+    assert torch.equal(again, Cm.data)
+    t.set_precision(0)
+    back = torch.zeros_like(Cm.data)
+    t.run(1.0, Cb=back, a_resident=True)
+    assert torch.equal(back, C64.data)
+
+
+def test_mixed_requires_dense_by_irrep(dev):
+    sp = bsp.Space.seeded(o=16, v=40, seed=3)
+    A = bsp.BSTensor(sp, "oovv", dev).fill_normal(1)
+    t = bsp.BlockTerm(bsp.BSTensor(sp, "oovv", dev), "ijab", A, "ikac", A, "kjcb", cfg=2)
+    with pytest.raises(ValueError):
+        t.set_precision(mixed.I8, 4, 5)
