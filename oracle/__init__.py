@@ -275,7 +275,8 @@ def dltc_program_run(table, arrays):
     multiplication C += coef * sum over the shared labels of A * B
     (task.cc:83-137 with compute.cc:3-36).  ``arrays`` maps every tensor
     name to an array (accumulators at zero) and is updated in place.  Pinned
-    by the brute-force loops of termlist_statement's test (same Einstein
+    by explicit loops over a two-layer table (tests/test_oracle.py) and by
+    the brute-force loops of termlist_statement's test (same Einstein
     sums)."""
     for layer in table["layers"]:
         for kind, i in layer:

@@ -439,3 +439,39 @@ def test_bs_contract_vs_dense_einsum():
                 continue
             sl = tuple(slice(starts[k][gi], starts[k][gi] + sp.counts[k][gi]) for k, gi in zip(ck, g))
             assert np.abs(dense[sl] - Cd[sl]).max(initial=0.0) < 1e-13
+
+
+def test_dltc_program_run_vs_loops():
+    """A two-layer driver table (an intermediate I built by a multiplication,
+    then read by a permuted scaled summation and by a second multiplication)
+    against explicit loops: layer order, coefficients, label permutation."""
+    rng = np.random.default_rng(9)
+    O, V = 2, 3
+    table = {
+        "tensors": {"T": ["PAR", "PAR", "HOL", "HOL"], "W": ["HOL", "HOL", "HOL", "HOL"],
+                    "X": ["HOL", "HOL"], "I_1": ["PAR", "PAR", "HOL", "HOL"],
+                    "R": ["PAR", "PAR", "HOL", "HOL"]},
+        "muls": [{"c": "I_1", "c_idx": ["p1", "p2", "h1", "h2"], "a": "T", "a_idx": ["p1", "p2", "h3", "h4"],
+                  "b": "W", "b_idx": ["h3", "h4", "h1", "h2"], "coef": 0.5},
+                 {"c": "R", "c_idx": ["p1", "p2", "h1", "h2"], "a": "I_1", "a_idx": ["p1", "p2", "h3", "h2"],
+                  "b": "X", "b_idx": ["h3", "h1"], "coef": -1.0}],
+        "sums": [{"c": "R", "c_idx": ["p1", "p2", "h1", "h2"], "a": "I_1", "a_idx": ["p2", "p1", "h2", "h1"],
+                  "coef": 2.0}],
+        "layers": [[["MUL", 0]], [["SUM", 0], ["MUL", 1]]],
+    }
+    T = rng.standard_normal((V, V, O, O))
+    W = rng.standard_normal((O, O, O, O))
+    X = rng.standard_normal((O, O))
+    arrays = {"T": T.copy(), "W": W.copy(), "X": X.copy(),
+              "I_1": np.zeros((V, V, O, O)), "R": np.zeros((V, V, O, O))}
+    oracle.dltc_program_run(table, arrays)
+    I = np.zeros((V, V, O, O))
+    for a, b, i, j, k, l in itertools.product(range(V), range(V), range(O), range(O), range(O), range(O)):
+        I[a, b, i, j] += 0.5 * T[a, b, k, l] * W[k, l, i, j]
+    R = np.zeros((V, V, O, O))
+    for a, b, i, j in itertools.product(range(V), range(V), range(O), range(O)):
+        R[a, b, i, j] += 2.0 * I[b, a, j, i]
+        for k in range(O):
+            R[a, b, i, j] += -1.0 * I[a, b, k, j] * X[k, i]
+    assert np.allclose(arrays["I_1"], I, rtol=0, atol=1e-13)
+    assert np.allclose(arrays["R"], R, rtol=0, atol=1e-13)
