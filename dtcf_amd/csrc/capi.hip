@@ -36,14 +36,19 @@ int pick_config(int64_t m, int64_t n, int64_t k)
 {
     const int forced = g_cfg.load();
     if (forced >= 0) return forced;
-    (void)k;
-    const int64_t tiles = ((m + 127) / 128) * ((n + 127) / 128);
-    if (tiles < 256) return kCfg64x64;
-    // thin results: a 128-wide tile would be mostly padding
-    if (n <= 64 || m <= 64) return kCfg64x64;
-    if (n <= 96) return kCfg192x96;
-    if (m <= 96) return kCfg96x192;
-    return kCfg128x128;
+    // Measured on gfx950 (view operands, every configuration forced in turn):
+    // with K <= 1024 the 64 x 64 tile wins or ties at every shape (9216 x
+    // 9216 x 192: 42.6 against 30.8 TF/s; 36864 x 2304 x 768: 54.4 / 54.0):
+    // a short K loop leaves the tile prologue and epilogue a large share,
+    // and two workgroups per CU overlap them.  Longer K takes 128 x 128
+    // unless its padding costs more than 12% over 64 x 64's (m = 192: 256
+    // padded rows; 1000 x 1000 x 100000: 63.6 against 50.7 TF/s, with the
+    // few tiles split over K).  192 x 96 and 96 x 192 were never the
+    // fastest (9216^3: 42.8 / 47.1 against 67.0 TF/s) and are used only
+    // when forced.
+    if (k <= 1024) return kCfg64x64;
+    auto padded = [&](int64_t t) { return (double)((m + t - 1) / t * t) * (double)((n + t - 1) / t * t); };
+    return padded(128) <= 1.12 * padded(64) ? kCfg128x128 : kCfg64x64;
 }
 
 }  // namespace dtcf

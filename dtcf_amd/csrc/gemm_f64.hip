@@ -143,7 +143,20 @@ struct TileLoader {
     }
 };
 
-// Interior fast path for dense operands: per-thread pointers advanced by BK
+// Rows past the operand's last (an edge tile) load the last row (the last
+// row pair for row-contiguous operands, whose row count is then even): in
+// range, and never stored, since the epilogue writes only rows < m and
+// columns < n.  Each C element still sums exactly its own row's and
+// column's products, so an edge tile's results equal the checked path's.
+template <int MODE>
+__device__ inline int64_t clamp_row(int64_t r, int64_t rows)
+{
+    constexpr bool KC = (MODE == kDK || MODE == kVK);
+    const int64_t last = KC ? rows - 1 : rows - 2;
+    return r < last ? r : last;
+}
+
+// Fast path for dense operands: per-thread pointers advanced by BK
 // (k-contiguous) or BK*ld (row-contiguous) each step, no bounds checks.
 template <int ROWS, int MODE>
 struct FastPtrs {
@@ -151,13 +164,14 @@ struct FastPtrs {
     const double *p[NP];
     int64_t step;
 
-    __device__ inline void init(const OpArgs &o, const double *base, int64_t row0, int w, int l)
+    __device__ inline void init(const OpArgs &o, const double *base, int64_t row0, int64_t rows,
+                                int w, int l)
     {
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             int row, k0, k1, l0, l1;
             TileLoader<ROWS, MODE>::coords(i, w, l, row, k0, k1, l0, l1);
-            p[i] = base + op_addr<MODE>(o, row0 + row, k0);
+            p[i] = base + op_addr<MODE>(o, clamp_row<MODE>(row0 + row, rows), k0);
         }
         step = (MODE == kDK) ? BK : BK * o.ld;
     }
@@ -170,7 +184,7 @@ struct FastPtrs {
     }
 };
 
-// Interior fast path for view operands whose block table folded away:
+// Fast path for view operands whose block table folded away:
 // (r, c) at rofs[r] + cofs[c].  Row offsets are read once; the column
 // offsets of the K step after next are fetched while the next step's data
 // loads are in flight, so no data load waits on a table load.
@@ -180,13 +194,13 @@ struct ViewFast {
     int64_t rb[NP], cn[NP];
     int kk[NP];
 
-    __device__ inline void init(const OpArgs &o, int64_t row0, int w, int l)
+    __device__ inline void init(const OpArgs &o, int64_t row0, int64_t rows, int w, int l)
     {
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             int row, k0, k1, l0, l1;
             TileLoader<ROWS, MODE>::coords(i, w, l, row, k0, k1, l0, l1);
-            rb[i] = o.rofs[row0 + row];
+            rb[i] = o.rofs[clamp_row<MODE>(row0 + row, rows)];
             kk[i] = k0;
         }
     }
@@ -246,8 +260,13 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, uint32_t q, int64_t
 
     constexpr bool DENSE = (AMODE <= kDMN) && (BMODE <= kDMN);
     constexpr bool VIEWS = (AMODE >= kVK) && (BMODE >= kVK);
-    const bool interior = g.a.vec2 && g.b.vec2 && (K % BK == 0) && (m0 + BM <= g.m) &&
-                          (n0 + BN <= g.n);
+    // Edge tiles take the fast paths too when their missing rows can be
+    // clamped (clamp_row): k-contiguous operands always, row-contiguous ones
+    // with an even row count (whole row pairs).
+    constexpr bool AKC = (AMODE == kDK || AMODE == kVK), BKC = (BMODE == kDK || BMODE == kVK);
+    const bool a_in = (m0 + BM <= g.m) || AKC || (g.m % 2 == 0);
+    const bool b_in = (n0 + BN <= g.n) || BKC || (g.n % 2 == 0);
+    const bool interior = g.a.vec2 && g.b.vec2 && (K % BK == 0) && a_in && b_in;
     // Uniform per workgroup: every load of this tile is in range and aligned.
     const bool fast = DENSE && interior;
     const bool vfast = VIEWS && interior && !g.a.blk && !g.b.blk;
@@ -257,13 +276,13 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, uint32_t q, int64_t
     ViewFast<BN, BMODE> vb;
     if (kt0 < kt1) {
         if (fast) {
-            fa.init(g.a, g.A, m0, wave, lane);
-            fb.init(g.b, g.B, n0, wave, lane);
+            fa.init(g.a, g.A, m0, g.m, wave, lane);
+            fb.init(g.b, g.B, n0, g.n, wave, lane);
             fa.load(la, kt0);
             fb.load(lb, kt0);
         } else if (vfast) {
-            va.init(g.a, m0, wave, lane);
-            vb.init(g.b, n0, wave, lane);
+            va.init(g.a, m0, g.m, wave, lane);
+            vb.init(g.b, n0, g.n, wave, lane);
             va.fetch_cols(g.a, kt0 * BK);
             vb.fetch_cols(g.b, kt0 * BK);
             va.load(la, g.A);

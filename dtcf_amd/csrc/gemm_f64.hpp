@@ -16,6 +16,8 @@ enum { kDK = 0, kDMN = 1, kVK = 2, kVMN = 3 };
 
 // Tile configurations (BM x BN), selectable per launch by the autotuner.
 enum { kCfg128x128 = 0, kCfg192x96 = 1, kCfg96x192 = 2, kCfg64x64 = 3, kCfgCount = 4 };
+// The C tile configuration dtcf_gemm_* uses for an m x n x k GEMM.
+int pick_config(int64_t m, int64_t n, int64_t k);
 
 struct OpArgs {
     int64_t ld;

@@ -1359,6 +1359,39 @@ extern "C" int dtcf_program_info(dtcf_program p, int64_t *block, int64_t *nst, d
     return DTCF_OK;
 }
 
+extern "C" int dtcf_program_statement_plan(dtcf_program p, int64_t i, int64_t *out)
+{
+    DTCF_REQUIRE(p && out, "program: null argument");
+    DTCF_REQUIRE(p->final, "program: statement plans exist after finalize");
+    DTCF_REQUIRE(i >= 0 && i < (int64_t)p->st.size(), "program: statement index out of range");
+    const PStmt &s = p->st[i];
+    for (int q = 0; q < DTCF_PLAN_NOUT; q++) out[q] = 0;
+    out[DTCF_PLAN_KIND] = s.kind;
+    if (s.kind == 1) {
+        out[DTCF_PLAN_NTERMS] = (int64_t)s.terms.size();
+        out[DTCF_PLAN_FLAGS] = s.plane ? DTCF_PLAN_SUM_PLANE : 0;
+        return DTCF_OK;
+    }
+    out[DTCF_PLAN_M] = s.m;
+    out[DTCF_PLAN_N] = s.n;
+    out[DTCF_PLAN_K] = s.k;
+    int64_t f = 0;
+    if (s.swap) f |= DTCF_PLAN_SWAP;
+    if (s.mixed) f |= DTCF_PLAN_MIXED;
+    if (s.pack[0]) f |= DTCF_PLAN_PACK_A;
+    if (s.pack[1]) f |= DTCF_PLAN_PACK_B;
+    if (s.av->colpairs) f |= DTCF_PLAN_A_COLPAIRS;
+    if (s.av->rowpairs) f |= DTCF_PLAN_A_ROWPAIRS;
+    if (s.bv->colpairs) f |= DTCF_PLAN_B_COLPAIRS;
+    if (s.bv->rowpairs) f |= DTCF_PLAN_B_ROWPAIRS;
+    if (!s.cv) f |= DTCF_PLAN_C_DENSE;
+    if (s.res[0]) f |= DTCF_PLAN_RES_A;
+    if (s.res[1]) f |= DTCF_PLAN_RES_B;
+    out[DTCF_PLAN_FLAGS] = f;
+    out[DTCF_PLAN_TILE] = s.mixed ? -1 : pick_config(s.m, s.n, s.k);
+    return DTCF_OK;
+}
+
 extern "C" int dtcf_program_destroy(dtcf_program p)
 {
     if (!p) return DTCF_OK;

@@ -53,6 +53,7 @@ def _declare():
         "dtcf_program_set_resident": [vp, C.c_char_p, C.c_int],
         "dtcf_program_get_layout": [vp, C.POINTER(C.c_int)],
         "dtcf_program_info": [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(dbl), C.POINTER(i64)],
+        "dtcf_program_statement_plan": [vp, i64, C.POINTER(i64)],
         "dtcf_program_destroy": [vp],
         "dtcf_request_submit": [vp, vp, C.POINTER(vp), C.POINTER(i64)],
         "dtcf_request_wait": [vp, i64, dbl],
@@ -298,6 +299,23 @@ class Program:
         lay = C.c_int()
         check(L.dtcf_program_get_layout(h, C.byref(lay)))
         self.layout = lay.value
+
+    PLAN_FLAGS = {1: "swap", 2: "mixed", 4: "pack_a", 8: "pack_b", 16: "a_colpairs",
+                  32: "a_rowpairs", 64: "b_colpairs", 128: "b_rowpairs", 256: "c_dense",
+                  512: "res_a", 1024: "res_b", 2048: "sum_plane"}
+    TILES = {-1: "mixed", 0: "128x128", 1: "192x96", 2: "96x192", 3: "64x64"}
+
+    def plan(self, i):
+        """The finalized plan of statement i (dtcf_program_statement_plan):
+        kind, GEMM m, n, k, tile configuration and access flags, or the
+        number of terms of a sum."""
+        out = (C.c_int64 * 8)()
+        check(lib().dtcf_program_statement_plan(self.h, int(i), out))
+        flags = [n for b, n in self.PLAN_FLAGS.items() if out[4] & b]
+        if out[0] == 1:
+            return {"kind": "sum", "nterms": out[6], "flags": flags}
+        return {"kind": "mul", "m": out[1], "n": out[2], "k": out[3], "flags": flags,
+                "tile": self.TILES.get(out[5], out[5])}
 
     def new_outputs(self, device="cuda"):
         return [torch.empty(int(np.prod(s)), dtype=torch.float64, device=device)

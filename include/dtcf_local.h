@@ -552,6 +552,25 @@ int dtcf_program_get_layout(dtcf_program p, int *layout);
  * contractions plus one per summed element. */
 int dtcf_program_info(dtcf_program p, int64_t *block_bytes, int64_t *nstatements, double *flops,
                       int64_t *noutputs);
+/* The plan of statement i of a finalized program (no reference counterpart:
+ * the reference has no planner to inspect): out[DTCF_PLAN_NOUT] receives
+ * the statement kind (0 contraction, 1 sum); for a contraction the GEMM
+ * m, n, k (operands possibly swapped), DTCF_PLAN_* flags (operand access:
+ * column or row pairs contiguous, packed into a dense copy first, resident
+ * slices; mixed precision; C written densely) and the FP64 C tile
+ * configuration (0 128x128, 1 192x96, 2 96x192, 3 64x64; -1 when mixed);
+ * for a sum the number of terms and whether it stages planes in LDS. */
+enum {
+    DTCF_PLAN_KIND = 0, DTCF_PLAN_M = 1, DTCF_PLAN_N = 2, DTCF_PLAN_K = 3, DTCF_PLAN_FLAGS = 4,
+    DTCF_PLAN_TILE = 5, DTCF_PLAN_NTERMS = 6, DTCF_PLAN_NOUT = 8
+};
+enum {
+    DTCF_PLAN_SWAP = 1, DTCF_PLAN_MIXED = 2, DTCF_PLAN_PACK_A = 4, DTCF_PLAN_PACK_B = 8,
+    DTCF_PLAN_A_COLPAIRS = 16, DTCF_PLAN_A_ROWPAIRS = 32, DTCF_PLAN_B_COLPAIRS = 64,
+    DTCF_PLAN_B_ROWPAIRS = 128, DTCF_PLAN_C_DENSE = 256, DTCF_PLAN_RES_A = 512,
+    DTCF_PLAN_RES_B = 1024, DTCF_PLAN_SUM_PLANE = 2048
+};
+int dtcf_program_statement_plan(dtcf_program p, int64_t i, int64_t *out);
 int dtcf_program_destroy(dtcf_program p);
 
 /* Requests: one evaluation of a finalized program.  `outputs` holds one

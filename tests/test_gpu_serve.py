@@ -343,3 +343,38 @@ def test_failed_statement_fails_its_request_only(dev, lists, nslots):
         for name in good:
             assert np.array_equal(again[name], good[name]), name
         p.destroy()
+
+
+def test_statement_plans(dev, lists):
+    """dtcf_program_statement_plan reports, for every statement of a
+    finalized T2 program, its kind, a GEMM shape whose m n k equals the
+    statement's external-by-external-by-contracted extents (either operand
+    order), a known tile configuration (FP64), and the mixed flag exactly on
+    the contractions the mixed-precision rule selects."""
+    tl = lists["t2"]
+    with S.Server(1 << 30, nslots=2) as srv:
+        sysm = S.System(srv, 0, 0, [tl], sizes=ASYM)
+        # mixed_min 2: the rule m n >= 2 (m + n) splits most contractions at these extents
+        progs = {0: S.Program(srv, tl, sysm),
+                 1: S.Program(srv, tl, sysm, prec=mixed.I8, nslice=4, mixed_min=2.0)}
+        for prec, prog in progs.items():
+            for i, s in enumerate(tl.statements):
+                pl = prog.plan(i)
+                assert pl["kind"] == s.kind
+                if s.kind == "sum":
+                    assert pl["nterms"] == len(s.terms)
+                    continue
+                a, b = s.terms
+                ext_a = [x for x in a.idx if x in s.lhs.idx]
+                ext_b = [x for x in b.idx if x in s.lhs.idx]
+                con = [x for x in a.idx if x in b.idx and x not in s.lhs.idx]
+                ea, eb, ek = (int(np.prod([s.extent(x, sysm.sizes) for x in g])) for g in (ext_a, ext_b, con))
+                assert pl["k"] == ek and sorted((pl["m"], pl["n"])) == sorted((ea, eb)), (s.lhs.name, pl)
+                mixed_rule = prec != 0 and pl["m"] * pl["n"] >= 2 * (pl["m"] + pl["n"])
+                assert ("mixed" in pl["flags"]) == mixed_rule, (s.lhs.name, pl)
+                assert pl["tile"] in (("mixed",) if mixed_rule else ("128x128", "64x64")), pl
+            if prec:
+                assert any("mixed" in prog.plan(i)["flags"] for i in range(len(tl.statements)))
+            prog.destroy()
+        with pytest.raises(Exception):
+            progs[0].plan(0)  # destroyed
