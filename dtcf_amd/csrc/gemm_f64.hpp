@@ -87,4 +87,13 @@ int gemm_view_overwrite(int64_t m, int64_t n, int64_t k, double alpha, const dou
                         dtcf_view av, const double *B, dtcf_view bv, double *C, dtcf_view cv,
                         int64_t ldc, hipStream_t stream);
 
+// Slice products (mixed.hip): dtcf_gemm_split_f64 with C = alpha * A * B^T
+// instead of C += when overwrite is set, which needs a fused kernel for
+// (prec, nslice, level): gemm_split_fused.
+bool gemm_split_fused(int prec, int ns, int lev);
+int gemm_split_f64_impl(int prec, int nslice, int64_t m, int64_t n, int64_t k, double alpha,
+                        const void *A_slices, const int32_t *ea, const void *B_slices,
+                        const int32_t *eb, int max_level, double *C, dtcf_view cv, int64_t ldc,
+                        void *stream, int overwrite);
+
 }  // namespace dtcf

@@ -747,6 +747,9 @@ struct OzArgs {
     ViewArgs cv;
     int64_t ldc;
     uint32_t tiles_m, tiles_n;
+    // 1: C = alpha * A B^T (the first write of each element stores, no read
+    // of C); 0: C += alpha * A B^T
+    int overwrite;
 };
 
 // Workgroup of 8 waves (2 per SIMD) as 4 x 2, each owning 32 x 64 of the
@@ -831,7 +834,8 @@ __device__ __forceinline__ void oz_flush(const OzArgs &g,
                 int64_t addr;
                 if constexpr (CVIEW) addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + cof;
                 else addr = row * g.ldc + cof;
-                g.C[addr] += g.alpha * v;
+                if (g.overwrite && first) g.C[addr] = g.alpha * v;
+                else g.C[addr] += g.alpha * v;
             }
     }
 }
@@ -1124,7 +1128,8 @@ __global__ __launch_bounds__(256) void oz_split_reduce(OzArgs g)
         if constexpr (CVIEW)
             addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + g.cv.cofs[col];
         else addr = row * g.ldc + col;
-        g.C[addr] += sum;
+        if (g.overwrite) g.C[addr] = sum;
+        else g.C[addr] += sum;
     }
 }
 
@@ -1181,6 +1186,20 @@ int oz_dispatch(int prec, int ns, int lev, const OzArgs &g, bool cview, hipStrea
 }
 
 }  // namespace
+
+// Whether (prec, ns, lev) has a fused slice-product kernel (the products
+// then support overwrite mode).
+bool gemm_split_fused(int prec, int ns, int lev)
+{
+    if (!g_fused_i8) return false;
+    if (prec == DTCF_PREC_BF16X3 || prec == DTCF_PREC_F16X3) return ns == 3 && lev == 4;
+    if (prec != DTCF_PREC_I8) return false;
+    switch (ns * 16 + lev) {
+    case 3 * 16 + 4: case 4 * 16 + 4: case 4 * 16 + 5: case 4 * 16 + 6: case 5 * 16 + 5:
+    case 5 * 16 + 6: return true;
+    default: return false;
+    }
+}
 }  // namespace dtcf
 
 using namespace dtcf;
@@ -1356,10 +1375,13 @@ extern "C" int dtcf_gemm_split_set_fused(int on)
     return DTCF_OK;
 }
 
-extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k,
-                                   double alpha, const void *A_slices, const int32_t *ea,
-                                   const void *B_slices, const int32_t *eb, int max_level,
-                                   double *C, dtcf_view cv, int64_t ldc, void *stream)
+namespace dtcf {
+// Slice products C (+)= alpha * A B^T; overwrite requires a fused kernel
+// (gemm_split_fused) and is refused otherwise.
+int gemm_split_f64_impl(int prec, int nslice, int64_t m, int64_t n, int64_t k, double alpha,
+                        const void *A_slices, const int32_t *ea, const void *B_slices,
+                        const int32_t *eb, int max_level, double *C, dtcf_view cv, int64_t ldc,
+                        void *stream, int overwrite)
 {
     DTCF_REQUIRE(m >= 0 && n >= 0 && k >= 0, "gemm_split: negative size");
     if (m == 0 || n == 0 || k == 0 || alpha == 0.0) return DTCF_OK;
@@ -1393,6 +1415,8 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
     // Slice pairs (i, j), 1-based, with i + j <= max_level (default ns + 1).
     const int lev = max_level > 0 ? max_level : ns + 1;
     if (lev < 2) return DTCF_OK;  // no pair has i + j <= 1: C is unchanged
+    DTCF_REQUIRE(!overwrite || gemm_split_fused(prec, ns, lev),
+                 "gemm_split: overwrite needs a fused slice-product kernel");
     hipStream_t s = as_stream(stream);
     if (prec != DTCF_PREC_F32 && g_fused_i8) {
         OzArgs o{};
@@ -1411,6 +1435,7 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
         o.ldc = ldc;
         o.tiles_m = (uint32_t)(g.Mp / OZ_BM);
         o.tiles_n = (uint32_t)(g.Np / OZ_BN);
+        o.overwrite = overwrite;
         // Level l = i + j sums the products q_i q_j, |q_1| <= 65, |q_i| <= 128
         // otherwise; INT32 stays exact while K times the largest level's
         // bound is below 2^31 (four digits to level 6: 49,408 per K, a flush
@@ -1479,4 +1504,14 @@ extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, i
             DTCF_HIP(hipGetLastError());
         }
     return DTCF_OK;
+}
+}  // namespace dtcf
+
+extern "C" int dtcf_gemm_split_f64(int prec, int nslice, int64_t m, int64_t n, int64_t k,
+                                   double alpha, const void *A_slices, const int32_t *ea,
+                                   const void *B_slices, const int32_t *eb, int max_level,
+                                   double *C, dtcf_view cv, int64_t ldc, void *stream)
+{
+    return gemm_split_f64_impl(prec, nslice, m, n, k, alpha, A_slices, ea, B_slices, eb, max_level,
+                               C, cv, ldc, stream, 0);
 }

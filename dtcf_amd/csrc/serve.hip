@@ -658,9 +658,15 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
         return gemm_view_overwrite(s.m, s.n, s.k, s.alpha, op[0], s.av, op[1], s.bv, c, s.cv, s.ldc,
                                    stream);
     }
-    // mixed precision accumulates its products into a cleared C
-    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(C.elems)), dim3(256), 0, stream, c, C.elems);
-    DTCF_HIP(hipGetLastError());
+    // mixed precision: the fused slice products write C (overwrite); the
+    // per-product path accumulates into a cleared C
+    const int ns_eff = p->prec == DTCF_PREC_I8 ? p->nslice : (p->prec == DTCF_PREC_F32 ? 1 : 3);
+    const int lev_eff = p->max_level > 0 ? p->max_level : ns_eff + 1;
+    const int overwrite = gemm_split_fused(p->prec, ns_eff, lev_eff) ? 1 : 0;
+    if (!overwrite) {
+        hipLaunchKernelGGL(zero_kernel, dim3(grid_for(C.elems)), dim3(256), 0, stream, c, C.elems);
+        DTCF_HIP(hipGetLastError());
+    }
     char *sl[2];
     int32_t *ex[2];
     bool split[2];
@@ -692,8 +698,8 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
     if (st == DTCF_OK && split[0]) st = dtcf_split_f64(p->prec, p->nslice, op[0], s.av, sa, ea, stream);
     if (st == DTCF_OK && split[1]) st = dtcf_split_f64(p->prec, p->nslice, op[1], s.bv, sb, eb, stream);
     if (st == DTCF_OK)
-        st = dtcf_gemm_split_f64(p->prec, p->nslice, s.m, s.n, s.k, s.alpha, sa, ea, sb, eb,
-                                 p->max_level, c, s.cv, s.ldc, stream);
+        st = gemm_split_f64_impl(p->prec, p->nslice, s.m, s.n, s.k, s.alpha, sa, ea, sb, eb,
+                                 p->max_level, c, s.cv, s.ldc, stream, overwrite);
     return st;
 }
 
