@@ -407,6 +407,98 @@ __global__ __launch_bounds__(256) void i8_digit_kernel(const double *__restrict_
     for (int p = 0; p < NS; p++) st_nt(out + p * plane16, wd[p][0], wd[p][1], wd[p][2], wd[p][3]);
 }
 
+// Three BF16 / FP16 slices of a table view (no block table), K-parallel:
+// 16 rows x 4 K steps (of 32 elements, 64 bytes) per workgroup, a wave per
+// K step; lane l = r + 16 kq converts row r's elements 8 kq .. 8 kq + 7 of
+// its step (its own lane of the fragment) and stores 16 bytes per slice
+// plane.  Column-pair views stage the step through LDS (16-byte loads of 4
+// rows x 256 contiguous bytes); other views read per lane.  FP16 rows are
+// scaled by 2^-rexp[row], rexp from i8_exp_kernel and f16_rexp_finish (the
+// frexp exponent of the row maximum less 14).  Same slices as split_kernel,
+// bit for bit.
+__global__ __launch_bounds__(256) void f16_rexp_finish(int32_t *__restrict__ rexp, int64_t rows_p)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < rows_p; r += (int64_t)gridDim.x * 256) {
+        const int e = rexp[r];
+        rexp[r] = (e == INT_MIN ? 0 : e) - 14;  // scaled rows have max |x| < 2^14
+    }
+}
+
+template <int MODE, bool PAIRS>
+__global__ __launch_bounds__(256) void f16_slices_kernel(const double *__restrict__ src, ViewArgs v,
+                                                         int64_t rows, int64_t K, int64_t rows_p,
+                                                         int64_t K_p, int64_t nrb,
+                                                         unsigned char *__restrict__ dst,
+                                                         const int32_t *__restrict__ rexp)
+{
+    static_assert(MODE == DTCF_PREC_BF16X3 || MODE == DTCF_PREC_F16X3, "two-byte slices");
+    constexpr int S = 34;  // doubles per staged row: 16-byte aligned, padded
+    __shared__ __attribute__((aligned(16))) double tile[PAIRS ? 4 : 1][PAIRS ? 16 * S : 2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, kq = lane >> 4;
+    const int64_t rb = (int64_t)blockIdx.x % nrb, ch = (int64_t)blockIdx.x / nrb, nk = K_p >> 5;
+    const int64_t kt = ch * 4 + w;
+    if (kt >= nk) return;  // whole waves leave: no workgroup barrier below
+    const int64_t row = rb * 16 + r;
+    double xs[8];
+    if constexpr (PAIRS) {
+        double *tw = tile[w];
+        const int cp = lane & 15, rq = lane >> 4;
+        const int64_t col = kt * 32 + 2 * cp;
+        const int64_t co = col < K ? v.cofs[col] : 0;
+        d2 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t rr = rb * 16 + 4 * u + rq;
+            a[u] = (rr < rows && col < K) ? ld_nt2(src + v.rofs[rr] + co) : d2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) *(d2 *)(tw + (4 * u + rq) * S + 2 * cp) = a[u];
+        // the tile is this wave's own: LDS accesses of one wave complete in
+        // order, so only the compiler must keep the reads below the writes
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+            const d2 b = *(const d2 *)(tw + r * S + 8 * kq + j);
+            xs[j] = b.x;
+            xs[j + 1] = b.y;
+        }
+    } else {
+        const int64_t c = kt * 32 + 8 * kq;
+        if (row < rows) {
+            const double *x = src + v.rofs[row];
+#pragma unroll
+            for (int j = 0; j < 8; j++) xs[j] = c + j < K ? ld_nt(x + v.cofs[c + j]) : 0.0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) xs[j] = 0.0;
+        }
+    }
+    const int e = MODE == DTCF_PREC_F16X3 ? rexp[row] : 0;
+    uint32_t wd[3][4] = {};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        double rr = MODE == DTCF_PREC_F16X3 ? ldexp(xs[j], -e) : xs[j];
+#pragma unroll
+        for (int sl = 0; sl < 3; sl++) {
+            uint16_t bits;
+            if constexpr (MODE == DTCF_PREC_BF16X3) {
+                const __bf16 q = (__bf16)(float)rr;  // nearest BF16 of the float of rr
+                bits = __builtin_bit_cast(uint16_t, q);
+                rr -= (double)(float)q;
+            } else {
+                const _Float16 q = (_Float16)(float)rr;
+                bits = __builtin_bit_cast(uint16_t, q);
+                rr -= (double)(float)q;
+            }
+            wd[sl][j >> 1] |= (uint32_t)bits << (16 * (j & 1));
+        }
+    }
+    const size_t plane16 = (size_t)rows_p * K_p * 2 / 16;
+    uint4 *out = (uint4 *)dst + (rb * nk + kt) * 64 + lane;
+#pragma unroll
+    for (int sl = 0; sl < 3; sl++) st_nt(out + sl * plane16, wd[sl][0], wd[sl][1], wd[sl][2], wd[sl][3]);
+}
+
 // Wide operands (few rows, long K: the T1 statements' A operands are a few
 // hundred rows by millions of K) leave the row-parallel split_kernel with a
 // few hundred waves.  There the float splits (and INT8 splits of views
@@ -1277,6 +1369,41 @@ extern "C" int dtcf_split_f64(int prec, int nslice, const double *src, dtcf_view
     const int64_t rp = round_up(rows, 128), kp = round_up(K, 128);
     const unsigned grid = (unsigned)((rp + 3) / 4);
     hipStream_t s = as_stream(stream);
+    // BF16 / FP16 slices of table views: K-parallel (FP16: row exponents first)
+    if ((prec == DTCF_PREC_BF16X3 || prec == DTCF_PREC_F16X3) && !v->blk) {
+        const bool pairs = v->colpairs && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+        const int64_t nrb = rp / 16, nd = nrb * ((kp / 32 + 3) / 4);
+        DTCF_REQUIRE(nd < (int64_t(1) << 31), "split: operand too large");
+        unsigned char *d1 = (unsigned char *)dst;
+        if (prec == DTCF_PREC_F16X3) {
+            const int64_t ne = nrb * ((K + 1023) / 1024);
+            DTCF_REQUIRE(ne < (int64_t(1) << 31), "split: operand too large");
+            const unsigned gi = (unsigned)std::min<int64_t>((rp + 255) / 256, 4096);
+            hipLaunchKernelGGL(i8_exp_init, dim3(gi), dim3(256), 0, s, rexp, rp);
+            if (pairs)
+                hipLaunchKernelGGL(i8_exp_kernel<true>, dim3((unsigned)ne), dim3(256), 0, s, src, a, rows, K,
+                                   nrb, rexp);
+            else
+                hipLaunchKernelGGL(i8_exp_kernel<false>, dim3((unsigned)ne), dim3(256), 0, s, src, a, rows,
+                                   K, nrb, rexp);
+            hipLaunchKernelGGL(f16_rexp_finish, dim3(gi), dim3(256), 0, s, rexp, rp);
+            if (pairs)
+                hipLaunchKernelGGL((f16_slices_kernel<DTCF_PREC_F16X3, true>), dim3((unsigned)nd), dim3(256), 0,
+                                   s, src, a, rows, K, rp, kp, nrb, d1, rexp);
+            else
+                hipLaunchKernelGGL((f16_slices_kernel<DTCF_PREC_F16X3, false>), dim3((unsigned)nd), dim3(256),
+                                   0, s, src, a, rows, K, rp, kp, nrb, d1, rexp);
+        } else {
+            if (pairs)
+                hipLaunchKernelGGL((f16_slices_kernel<DTCF_PREC_BF16X3, true>), dim3((unsigned)nd), dim3(256),
+                                   0, s, src, a, rows, K, rp, kp, nrb, d1, rexp);
+            else
+                hipLaunchKernelGGL((f16_slices_kernel<DTCF_PREC_BF16X3, false>), dim3((unsigned)nd),
+                                   dim3(256), 0, s, src, a, rows, K, rp, kp, nrb, d1, rexp);
+        }
+        DTCF_HIP(hipGetLastError());
+        return DTCF_OK;
+    }
     // INT8 table views (no block table) split in two K-parallel passes at
     // any shape: exponents, then digits.
     if (prec == DTCF_PREC_I8 && !v->blk) {

@@ -207,13 +207,15 @@ def test_fused_int8_flush_interval_at_largest_digits(dev, ns, lev):
 
 
 @pytest.mark.parametrize("prec", [mixed.F32, mixed.BF16X3, mixed.F16X3])
-@pytest.mark.parametrize("rows,K", [(40, 20000), (300, 520)])
+@pytest.mark.parametrize("rows,K", [(40, 20000), (300, 520), (600, 3000), (17, 66)])
 def test_float_split_bitwise_vs_host(dev, prec, rows, K):
     """FP32 / BF16x3 / FP16x3 slices decoded from their planes equal a host
     emulation bit for bit: each slice is the FP64 remainder rounded once to
     the nearest BF16 / FP16 (ties to even), FP16 rows first scaled by
-    2^-(e-14), e the exponent of the row max.  40 x 20000 takes the
-    K-parallel launches, 300 x 520 the row-parallel kernel."""
+    2^-(e-14), e the exponent of the row max.  BF16 / FP16 slices of the
+    row-major (column-pair) and column-major (row-run) views take the
+    K-parallel slice kernels; FP32 40 x 20000 the wide launches, the other
+    FP32 shapes the row-parallel kernel."""
     rng = np.random.default_rng(rows + K + prec)
     A = rng.standard_normal((rows, K)) * np.exp2(rng.integers(-8, 8, size=(rows, 1)))
     A[1] = 0.0
