@@ -19,6 +19,11 @@
 //   PACKED : permute+pack in one gather pass per operand, GEMM, one scatter;
 //   FUSED  : a single GEMM that reads A, B and C straight from the tiles
 //            through offset tables (no big matrices at all).
+// An A or B operand of a symmetric tensor whose tiles carry several composed
+// maps (bounced copies next to stored tiles) cannot be one offset table: in
+// PACKED and FUSED it is packed by one permute pass per class, each writing
+// its tiles straight into their blocks of the big matrix, and FUSED's GEMM
+// reads that big matrix densely.  C is never bounced (one class).
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -238,6 +243,81 @@ int make_view(const OperandPlan &P, bool packed_tiles, dtcf_view *out)
     return dtcf_view_create(rows, cols, rofs.data(), cofs.data(), blk.data(), P.rpb, P.cpb, out);
 }
 
+// One permute pass of a symmetric operand: the tiles of one bounce class,
+// each permuted by the class's composed map and written into its block of
+// the row-major big matrix (row pitch ld), as create_big_matrix places
+// permuted tiles (transpose_helper.cc:149-159).
+struct ClassPack {
+    dtcf_strided d{};
+    int64_t n = 0;
+    int64_t *src = nullptr, *dst = nullptr;  // device offset tables
+};
+
+int make_class_packs(const OperandPlan &P, int64_t ld, std::vector<ClassPack> &out)
+{
+    const int D = P.dims;
+    int64_t istr[DTCF_MAX_DIMS];
+    istr[D - 1] = 1;
+    for (int i = D - 2; i >= 0; i--) istr[i] = istr[i + 1] * P.w[i + 1];
+    for (size_t c = 0; c < P.classes.size(); c++) {
+        const std::vector<int> &comp = P.classes[c];
+        int64_t pw[DTCF_MAX_DIMS], pstr[DTCF_MAX_DIMS];
+        for (int i = 0; i < D; i++) pw[comp[i]] = P.w[i];
+        // positions < nrow_dims form the block's rows (pitch ld), the rest
+        // its columns; the last position of each group is the fastest
+        int64_t acc = 1;
+        for (int p = D - 1; p >= P.nrow_dims; p--) {
+            pstr[p] = acc;
+            acc *= pw[p];
+        }
+        const int64_t cols = acc;
+        acc = ld;
+        for (int p = P.nrow_dims - 1; p >= 0; p--) {
+            pstr[p] = acc;
+            acc *= pw[p];
+        }
+        // a swap of two dims of different extents across the row/column
+        // split would not fill the rpb x cpb block: refuse it
+        DTCF_REQUIRE(cols == P.cpb && acc == ld * P.rpb,
+                     "step: bounced dims of different extents change the block shape");
+        ClassPack k;
+        k.d.dims = D;
+        for (int i = 0; i < D; i++) {
+            k.d.ext[i] = P.w[i];
+            k.d.istr[i] = istr[i];
+            k.d.ostr[i] = pstr[comp[i]];
+        }
+        std::vector<int64_t> so, dso;
+        for (int64_t b = 0; b < P.nrb * P.ncb; b++) {
+            const int64_t t = P.order[b];
+            if (P.cls[t] != (int)c) continue;
+            so.push_back(t * P.bs);
+            dso.push_back((b / P.ncb) * P.rpb * ld + (b % P.ncb) * P.cpb);
+        }
+        k.n = (int64_t)so.size();
+        out.push_back(k);  // owned by `out` from here (freed with the step)
+        if (k.n == 0) continue;
+        ClassPack &o = out.back();
+        hipError_t e = hipMalloc(&o.src, k.n * sizeof(int64_t));
+        if (e == hipSuccess) e = hipMalloc(&o.dst, k.n * sizeof(int64_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(o.src, so.data(), k.n * sizeof(int64_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(o.dst, dso.data(), k.n * sizeof(int64_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "step: class pack offsets");
+    }
+    return DTCF_OK;
+}
+
+// Dense row-major rows x cols view (a packed big matrix read by FUSED).
+int make_dense_view(int64_t rows, int64_t cols, dtcf_view *out)
+{
+    std::vector<int64_t> rofs(rows), cofs(cols);
+    for (int64_t r = 0; r < rows; r++) rofs[r] = r * cols;
+    for (int64_t c = 0; c < cols; c++) cofs[c] = c;
+    return dtcf_view_create(rows, cols, rofs.data(), cofs.data(), nullptr, 0, 0, out);
+}
+
 }  // namespace
 }  // namespace dtcf
 
@@ -254,6 +334,8 @@ struct dtcf_step_s {
     int64_t tmp_elems = 0;
     // per-class source/destination offsets for the LITERAL permute passes
     std::vector<int64_t *> cls_off;  // device arrays
+    // PACKED / FUSED: A or B packed class by class (several bounce classes)
+    std::vector<ClassPack> packA, packB;
     bool empty = false;
     // Optional timing of the GEMM kernel: one event pair per run, recorded on
     // the stream the GEMM is launched on.
@@ -279,6 +361,11 @@ extern "C" int dtcf_step_destroy(dtcf_step s)
         if (p) (void)hipFree(p);
     for (int64_t *p : s->cls_off)
         if (p) (void)hipFree(p);
+    for (auto *v : {&s->packA, &s->packB})
+        for (ClassPack &k : *v) {
+            if (k.src) (void)hipFree(k.src);
+            if (k.dst) (void)hipFree(k.dst);
+        }
     for (auto &e : s->ev) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -384,13 +471,22 @@ extern "C" int dtcf_step_plan(const dtcf_tileset *A, const char *a_idx, const dt
     if (s->C.nrb * s->C.rpb != s->M || s->C.ncb * s->C.cpb != s->N)
         return fail("step: C big matrix is not (A rows) x (B rows)");
 
-    const bool single = s->A.classes.size() == 1 && s->B.classes.size() == 1 &&
-                        s->C.classes.size() == 1;
-    if (mode != DTCF_STEP_LITERAL && !single)
-        return fail("step: symmetric tiles with several bounce permutations need LITERAL mode");
+    if (s->C.classes.size() != 1) return fail("step: C tiles with several permutations");
+    const bool multiA = s->A.classes.size() > 1, multiB = s->B.classes.size() > 1;
+    if (mode != DTCF_STEP_LITERAL && (multiA || multiB)) {
+        if ((multiA && ((st = make_class_packs(s->A, s->K, s->packA)) != DTCF_OK ||
+                        (st = step_alloc(&s->bigA, s->M * s->K)) != DTCF_OK)) ||
+            (multiB && ((st = make_class_packs(s->B, s->K, s->packB)) != DTCF_OK ||
+                        (st = step_alloc(&s->bigB, s->N * s->K)) != DTCF_OK))) {
+            dtcf_step_destroy(s);
+            return st;
+        }
+    }
     if (mode == DTCF_STEP_FUSED) {
-        if ((st = make_view(s->A, false, &s->va)) != DTCF_OK ||
-            (st = make_view(s->B, false, &s->vb)) != DTCF_OK ||
+        if ((st = multiA ? make_dense_view(s->M, s->K, &s->va) : make_view(s->A, false, &s->va)) !=
+                DTCF_OK ||
+            (st = multiB ? make_dense_view(s->N, s->K, &s->vb) : make_view(s->B, false, &s->vb)) !=
+                DTCF_OK ||
             (st = make_view(s->C, false, &s->vc)) != DTCF_OK) {
             dtcf_step_destroy(s);
             return st;
@@ -400,8 +496,8 @@ extern "C" int dtcf_step_plan(const dtcf_tileset *A, const char *a_idx, const dt
         if ((st = make_view(s->A, lit, &s->va)) != DTCF_OK ||
             (st = make_view(s->B, lit, &s->vb)) != DTCF_OK ||
             (st = make_view(s->C, lit, &s->vc)) != DTCF_OK ||
-            (st = step_alloc(&s->bigA, s->M * s->K)) != DTCF_OK ||
-            (st = step_alloc(&s->bigB, s->N * s->K)) != DTCF_OK ||
+            (!s->bigA && (st = step_alloc(&s->bigA, s->M * s->K)) != DTCF_OK) ||
+            (!s->bigB && (st = step_alloc(&s->bigB, s->N * s->K)) != DTCF_OK) ||
             (st = step_alloc(&s->bigC, s->M * s->N)) != DTCF_OK) {
             dtcf_step_destroy(s);
             return st;
@@ -539,8 +635,9 @@ static int gemm_timed(dtcf_step s, hipStream_t st, double alpha)
     if (e) DTCF_HIP(hipEventRecord(e[0], st));
     int r;
     if (s->mode == DTCF_STEP_FUSED)
-        r = dtcf_gemm_view_f64(s->M, s->N, s->K, alpha, s->A.tiles, s->va, s->B.tiles, s->vb,
-                               s->C.tiles, s->vc, 0, st);
+        r = dtcf_gemm_view_f64(s->M, s->N, s->K, alpha, s->packA.empty() ? s->A.tiles : s->bigA,
+                               s->va, s->packB.empty() ? s->B.tiles : s->bigB, s->vb, s->C.tiles,
+                               s->vc, 0, st);
     else
         r = dtcf_gemm_f64(0, 1, s->M, s->N, s->K, alpha, s->bigA, s->K, s->bigB, s->K, s->bigC,
                           s->N, st);
@@ -558,11 +655,29 @@ extern "C" int dtcf_step_run(dtcf_step s, double alpha, void *stream)
     if (s->empty) return DTCF_OK;
     hipStream_t st = as_stream(stream);
     int r;
-    if (s->mode == DTCF_STEP_FUSED) return gemm_timed(s, st, alpha);
+    // symmetric A / B: one permute pass per bounce class into the big matrix
+    auto class_packs = [&](const OperandPlan &P, std::vector<ClassPack> &packs, double *big) {
+        for (const ClassPack &k : packs) {
+            if (k.n == 0) continue;
+            const int q = dtcf_permute_copy_f64(P.tiles, big, &k.d, k.n, k.src, 0, k.dst, 0, 1.0,
+                                                0.0, st);
+            if (q != DTCF_OK) return q;
+        }
+        return (int)DTCF_OK;
+    };
+    if (s->mode == DTCF_STEP_FUSED) {
+        if ((r = class_packs(s->B, s->packB, s->bigB)) != DTCF_OK) return r;
+        if ((r = class_packs(s->A, s->packA, s->bigA)) != DTCF_OK) return r;
+        return gemm_timed(s, st, alpha);
+    }
     if (s->mode == DTCF_STEP_PACKED) {
         if ((r = dtcf_view_gather_f64(s->C.tiles, s->vc, s->bigC, s->N, st)) != DTCF_OK) return r;
-        if ((r = dtcf_view_gather_f64(s->B.tiles, s->vb, s->bigB, s->K, st)) != DTCF_OK) return r;
-        if ((r = dtcf_view_gather_f64(s->A.tiles, s->va, s->bigA, s->K, st)) != DTCF_OK) return r;
+        if ((r = s->packB.empty() ? dtcf_view_gather_f64(s->B.tiles, s->vb, s->bigB, s->K, st)
+                                  : class_packs(s->B, s->packB, s->bigB)) != DTCF_OK)
+            return r;
+        if ((r = s->packA.empty() ? dtcf_view_gather_f64(s->A.tiles, s->va, s->bigA, s->K, st)
+                                  : class_packs(s->A, s->packA, s->bigA)) != DTCF_OK)
+            return r;
         if ((r = gemm_timed(s, st, alpha)) != DTCF_OK) return r;
         return dtcf_view_scatter_f64(s->bigC, s->N, s->vc, s->C.tiles, st);
     }

@@ -128,9 +128,11 @@ static void views_and_gemms()
     hipFree(dG);
 }
 
-static void rrr_step()
+static void rrr_step(bool sym)
 {
-    // ring C[i,j,a,b] += A[i,k,a,c] B[k,j,c,b], 2 x 2 x 2 x 2 tiles of 2 x 2 x 3 x 3
+    // ring C[i,j,a,b] += A[i,k,a,c] B[k,j,c,b], 2 x 2 x 2 x 2 tiles of 2 x 2 x 3 x 3;
+    // with `sym`, A and B are symmetric in their first two dims, so their
+    // tiles with addr0 < addr1 are bounced copies (two classes per operand)
     const int dims = 4, nt = 16, tsz = 2 * 2 * 3 * 3;
     std::vector<int32_t> addr(nt * dims);
     for (int t = 0; t < nt; t++)
@@ -149,7 +151,7 @@ static void rrr_step()
             for (int d = 0; d < dims; d++) {
                 t->vgrid[d] = 2;
                 t->block_range[d] = d < 2 ? 2 : 3;
-                t->sym_group[d] = 3;
+                t->sym_group[d] = (sym && t != &tc && d < 2) ? 0 : 3;
             }
             t->ntiles = nt;
             t->tile_addr = addr.data();
@@ -466,7 +468,8 @@ int main()
     }
     views_and_gemms();
     std::printf("views and gemms done\n");
-    rrr_step();
+    rrr_step(false);
+    rrr_step(true);
     std::printf("rrr step done\n");
     dltc_terms();
     std::printf("dltc done\n");

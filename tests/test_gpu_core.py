@@ -314,18 +314,13 @@ def test_local_step_ladder_and_transposed_output(dev):
         assert relerr(got, ref) <= GEMM_TOL, mode
 
 
-def test_local_step_symmetric_bounce_literal(dev):
-    # A is permutationally symmetric in (i,k) ('aacc').  Only tiles with
-    # addr0 >= addr1 are stored; a bounced copy arrives under the swapped
-    # address carrying the stored tile's data, and the plan must swap its
-    # dims back without a sign (get_symm_permutation, contraction.cc:132-160).
-    o, v = 6, 8
-    vg = [3, 3, 2, 2]
-    rng = np.random.default_rng(9)
-    X = rng.standard_normal((o, o, v, v))
-    full = X + X.transpose(1, 0, 2, 3)
-    stored = rrr.tile_addresses("aacc", vg)
-    br = [o // 3, o // 3, v // 2, v // 2]
+def _bounced(sym, full, vg, dev):
+    """Tiles of a tensor symmetric in its first two dims: only tiles with
+    addr0 >= addr1 are stored; a bounced copy arrives under the swapped
+    address carrying the stored tile's data, and the plan must swap its dims
+    back without a sign (get_symm_permutation, contraction.cc:132-160)."""
+    stored = rrr.tile_addresses(sym, vg)
+    br = [full.shape[d] // vg[d] for d in range(4)]
     addrs, datas = [], []
     for a in stored:
         sl = tuple(slice(a[d] * br[d], (a[d] + 1) * br[d]) for d in range(4))
@@ -335,20 +330,57 @@ def test_local_step_symmetric_bounce_literal(dev):
         if a[0] > a[1]:
             addrs.append([a[1], a[0], a[2], a[3]])
             datas.append(blk)  # same data under the bounced address
-    tA = rrr.TileTensor("aacc", full.shape, vg, dev,
-                        data=torch.from_numpy(np.concatenate(datas)).to(dev), addrs=addrs)
+    return rrr.TileTensor(sym, full.shape, vg, dev,
+                          data=torch.from_numpy(np.concatenate(datas)).to(dev), addrs=addrs)
+
+
+@pytest.mark.parametrize("mode", [rrr.LITERAL, rrr.PACKED, rrr.FUSED])
+def test_local_step_symmetric_bounce(dev, mode):
+    # A is permutationally symmetric in (i,k) ('aacc'): two bounce classes.
+    # LITERAL permutes class by class into a buffer and gathers; PACKED and
+    # FUSED pack each class straight into the big matrix.
+    o, v = 6, 8
+    vg = [3, 3, 2, 2]
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((o, o, v, v))
+    full = X + X.transpose(1, 0, 2, 3)
+    tA = _bounced("aacc", full, vg, dev)
     B = rng.standard_normal((o, o, v, v))
     tB = rrr.TileTensor.from_dense("cccc", B, vg, dev)
     Cm = rng.standard_normal((o, o, v, v))
     tC = rrr.TileTensor.from_dense("cccc", Cm, vg, dev)
-    st = rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=rrr.LITERAL)
+    say("symmetric bounce, mode", mode)
+    st = rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=mode)
     st.run(1.0)
     got = rrr.dense_from_tiles(tC.tiles.cpu().numpy(), Cm.shape, vg, tC.addrs)
     ref = Cm + np.einsum("ikac,kjcb->ijab", full, B)
     assert relerr(got, ref) <= GEMM_TOL
-    # the fused modes refuse several bounce classes rather than mis-read them
-    with pytest.raises(Exception):
-        rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=rrr.FUSED)
+
+
+@pytest.mark.parametrize("mode", [rrr.LITERAL, rrr.PACKED, rrr.FUSED])
+def test_local_step_both_operands_symmetric(dev, mode):
+    # A symmetric in (i,k), B in (k,j) ('aacc' both), uneven tile extents
+    # (o=9 in 3 tiles, v=10 in 2) and the contraction index k split over
+    # row and column roles of the two operands: both bounced operands are
+    # packed by class, C is read through its view.
+    o, v = 9, 10
+    vg = [3, 3, 2, 2]
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((o, o, v, v))
+    fa = X + X.transpose(1, 0, 2, 3)
+    Y = rng.standard_normal((o, o, v, v))
+    fb = Y + Y.transpose(1, 0, 2, 3)
+    tA = _bounced("aacc", fa, vg, dev)
+    tB = _bounced("aacc", fb, vg, dev)
+    Cm = rng.standard_normal((o, o, v, v))
+    tC = rrr.TileTensor.from_dense("cccc", Cm, vg, dev)
+    say("both symmetric, mode", mode)
+    st = rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=mode)
+    for _ in range(2):  # a second run re-packs over the first run's big matrices
+        st.run(0.5)
+    got = rrr.dense_from_tiles(tC.tiles.cpu().numpy(), Cm.shape, vg, tC.addrs)
+    ref = Cm + np.einsum("ikac,kjcb->ijab", fa, fb)
+    assert relerr(got, ref) <= GEMM_TOL
 
 
 @pytest.mark.parametrize("o,v", [(6, 10), (5, 7), (12, 70)])
