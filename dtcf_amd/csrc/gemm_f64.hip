@@ -223,6 +223,19 @@ struct Occ {
     static constexpr int v = (BM * BN <= 192 * 96) ? 2 : 1;
 };
 
+// Logical tile q -> (tm, tn) through a grouped (8 tile rows) swizzle.
+__device__ inline void tile_coords(uint32_t q, uint32_t tiles_m, uint32_t tiles_n, uint32_t &tm,
+                                   uint32_t &tn)
+{
+    const uint32_t GROUP = 8;
+    const uint32_t width = GROUP * tiles_n;
+    const uint32_t gid = q / width;
+    const uint32_t first = gid * GROUP;
+    const uint32_t gsz = min(tiles_m - first, GROUP);
+    tm = first + (q % width) % gsz;
+    tn = (q % width) / gsz;
+}
+
 // One workgroup's share of a GEMM: logical tile q over K steps [kt0, kt1);
 // a split slice (split) writes raw partial sums to workspace slot sidx.
 template <int BM, int BN, int WM, int WN, int AMODE, int BMODE, bool CVIEW>
@@ -240,14 +253,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, uint32_t q, int64_t
     const int wm = wave / (BN / WN), wn = wave % (BN / WN);
     const int64_t K = g.k;
 
-    // Logical tile q -> (tm, tn) through a grouped (8 tile rows) swizzle.
-    const uint32_t GROUP = 8;
-    const uint32_t width = GROUP * g.tiles_n;
-    const uint32_t gid = q / width;
-    const uint32_t first = gid * GROUP;
-    const uint32_t gsz = min(g.tiles_m - first, GROUP);
-    const uint32_t tm = first + (q % width) % gsz;
-    const uint32_t tn = (q % width) / gsz;
+    uint32_t tm, tn;
+    tile_coords(q, g.tiles_m, g.tiles_n, tm, tn);
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
 
     TileLoader<BM, AMODE> la;
@@ -417,23 +424,57 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_kernel(GemmArgs
     gemm_tile<BM, BN, WM, WN, AMODE, BMODE, CVIEW>(g, q, kt0, kt1, split, sidx);
 }
 
+// Problem owning tail tile ti (tail tiles are numbered problem by problem).
+__device__ inline int tail_problem(const GemmGroupArgs &ga, uint32_t ti)
+{
+    int p = 0;
+    for (int i = 0; i < ga.np; i++) {
+        const GemmGroupProb &P = ga.p[i];
+        if (ti >= P.tail0 && ti < P.tail0 + (P.tiles_m * P.tiles_n - P.t_whole)) p = i;
+    }
+    return p;
+}
+
 // Several independent dense GEMMs C_p (+)= alpha A_p B_p^T in one launch.
-// Problem p owns workgroups [wg0_p, wg0_p + tiles_p rounded up to 8); wg0_p
-// is a multiple of 8, so a workgroup's XCD is its local id mod 8 and each
-// problem gets the same XCD-contiguous tile order as a lone launch.
-// Workgroups past a problem's last tile exit at once.
-template <int BM, int BN, int WM, int WN>
+// Problem p's whole tiles own workgroups [wg0_p, wg0_p + t_whole_p rounded
+// up to 8); wg0_p is a multiple of 8, so a workgroup's XCD is its local id
+// mod 8 and each problem gets the same XCD-contiguous tile order as a lone
+// launch.  (Cutting the list of all whole tiles into eight runs of equal
+// work, one per XCD, was measured slower: 8 x 2048^3 55.0 against 56.6
+// TF/s, 2 x 4096 x 4096 x 2048 59.0 against 63.7.)  Workgroups past a
+// problem's last whole tile exit at once.  The workgroups after wg_whole
+// each own one K slice of a tail tile (TAIL: a launch without tail tiles
+// compiles the slice path out, which keeps the 192 x 96 tiles' register
+// spills at their level without it).
+template <int BM, int BN, int WM, int WN, bool TAIL>
 __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_group_kernel(GemmGroupArgs ga)
 {
     const uint32_t bid = blockIdx.x;
     int p = 0;
-    for (int i = 1; i < ga.np; i++)
-        if (bid >= ga.p[i].wg0) p = i;
+    uint32_t q, sidx = 0;
+    bool split = false;
+    if (!TAIL || bid < ga.wg_whole) {
+        for (int i = 1; i < ga.np; i++)
+            if (bid >= ga.p[i].wg0) p = i;
+        const GemmGroupProb &P = ga.p[p];
+        const uint32_t T = P.t_whole, T8 = (T + 7u) & ~7u;
+        const uint32_t local = bid - P.wg0;
+        q = (local & 7u) * (T8 >> 3) + (local >> 3);
+        if (q >= T) return;  // whole workgroup: no barrier has been reached
+    } else {
+        split = true;
+        sidx = bid - ga.wg_whole;
+        const uint32_t ti = sidx / ga.sk;
+        p = tail_problem(ga, ti);
+        q = ga.p[p].t_whole + (ti - ga.p[p].tail0);
+    }
     const GemmGroupProb &P = ga.p[p];
-    const uint32_t T = P.tiles_m * P.tiles_n, T8 = (T + 7u) & ~7u;
-    const uint32_t local = bid - P.wg0;
-    const uint32_t q = (local & 7u) * (T8 >> 3) + (local >> 3);
-    if (q >= T) return;  // whole workgroup: no barrier has been reached
+    const int64_t nk = (P.k + BK - 1) / BK;
+    int64_t kt0 = 0, kt1 = nk;
+    if (split) {
+        kt0 = min(nk, (int64_t)(sidx % ga.sk) * P.kchunk);
+        kt1 = min(nk, kt0 + P.kchunk);
+    }
     GemmArgs g{};
     g.m = P.m;
     g.n = P.n;
@@ -450,7 +491,29 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_group_kernel(Ge
     g.tiles_m = P.tiles_m;
     g.tiles_n = P.tiles_n;
     g.overwrite = ga.overwrite;
-    gemm_tile<BM, BN, WM, WN, kDK, kDK, false>(g, q, 0, (P.k + BK - 1) / BK, false, 0);
+    g.ws = ga.ws;
+    gemm_tile<BM, BN, WM, WN, kDK, kDK, false>(g, q, kt0, kt1, TAIL && split, sidx);
+}
+
+// Sums the K slices of each tail tile of a group in slice order and adds
+// alpha * sum to (or, overwriting, stores it in) the problem's C.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void group_split_reduce_kernel(GemmGroupArgs ga)
+{
+    const uint32_t ti = blockIdx.y;
+    const GemmGroupProb &P = ga.p[tail_problem(ga, ti)];
+    uint32_t tm, tn;
+    tile_coords(P.t_whole + (ti - P.tail0), P.tiles_m, P.tiles_n, tm, tn);
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < BM * BN; e += gridDim.x * 256) {
+        const int64_t row = (int64_t)tm * BM + e / BN, col = (int64_t)tn * BN + e % BN;
+        if (row >= P.m || col >= P.n) continue;
+        double sum = 0.0;
+        const double *w = ga.ws + (int64_t)ti * ga.sk * BM * BN + e;
+        for (uint32_t sl = 0; sl < ga.sk; sl++) sum += w[(int64_t)sl * BM * BN];
+        const int64_t addr = row * P.ldc + col;
+        if (ga.overwrite) P.C[addr] = ga.alpha * sum;
+        else P.C[addr] += ga.alpha * sum;
+    }
 }
 
 // Sums the K slices of each remainder tile in slice order and adds
@@ -459,14 +522,8 @@ template <int BM, int BN, bool CVIEW>
 __global__ __launch_bounds__(256) void split_reduce_kernel(GemmArgs g)
 {
     const uint32_t t = blockIdx.y;  // remainder tile
-    const uint32_t q = g.t_full + t;
-    const uint32_t GROUP = 8;
-    const uint32_t width = GROUP * g.tiles_n;
-    const uint32_t gid = q / width;
-    const uint32_t first = gid * GROUP;
-    const uint32_t gsz = min(g.tiles_m - first, GROUP);
-    const uint32_t tm = first + (q % width) % gsz;
-    const uint32_t tn = (q % width) / gsz;
+    uint32_t tm, tn;
+    tile_coords(g.t_full + t, g.tiles_m, g.tiles_n, tm, tn);
     for (int e = blockIdx.x * 256 + threadIdx.x; e < BM * BN; e += gridDim.x * 256) {
         const int64_t row = (int64_t)tm * BM + e / BN, col = (int64_t)tn * BN + e % BN;
         if (row >= g.m || col >= g.n) continue;
@@ -531,6 +588,41 @@ int split_workspace(hipStream_t s, size_t need, double **ws)
     return DTCF_OK;
 }
 
+// How many of T tiles to cut into K slices (nsplit, the last ones) and
+// into how many (sk >= 2; sk < 2 means no split), for `slots` resident
+// workgroups and nk K steps per tile.  A last partial round of at most half
+// the slots: its tiles are cut to fill one round.  Under-filled grid (at
+// most half the slots): every tile is cut.  One partial round (more than
+// half the slots, at most all): every tile is cut into the sk <= 8 slices
+// that fill whole rounds best, within two rounds of slots, when that beats
+// the unsplit fill by 3% (a 340-tile GEMM on 512 slots: 3 slices, 1020
+// items in 2 rounds, 66% -> 99.6% of the slots busy).  In every case
+// nsplit * sk <= 2 * slots, and every slice is non-empty.
+void tail_split(uint32_t T, uint32_t slots, int64_t nk, uint32_t &nsplit, uint32_t &sk)
+{
+    const uint32_t rem = T % slots;
+    nsplit = 0;
+    sk = 0;
+    if (T > slots && rem > 0 && rem <= slots / 2 && nk >= 32) {
+        nsplit = rem;
+        sk = std::min<uint32_t>(slots / rem, (uint32_t)(nk / 8));
+    } else if (T <= slots / 2 && nk >= 16) {
+        nsplit = T;
+        sk = std::min<uint32_t>(slots / T, (uint32_t)(nk / 4));
+    } else if (T <= slots && nk >= 32) {
+        double best = (double)T / slots;
+        for (uint32_t c = 2; c <= 8 && nk / c >= 8 && T * c <= 2 * slots; c++) {
+            const uint32_t items = T * c;
+            const double fill = (double)items / (double)(((items + slots - 1) / slots) * slots);
+            if (fill > best * 1.03) {
+                best = fill;
+                sk = c;
+            }
+        }
+        if (sk >= 2) nsplit = T;
+    }
+}
+
 template <int BM, int BN, int WM, int WN, int AM, int BMD, bool CV>
 int launch_one(GemmArgs g, hipStream_t s)
 {
@@ -553,36 +645,11 @@ int launch_one(GemmArgs g, hipStream_t s)
     const uint32_t T = g.tiles_m * g.tiles_n;
     const uint32_t slots = (uint32_t)cus * (uint32_t)Occ<BM, BN>::v;
     const int64_t nk = (g.k + BK - 1) / BK;
-    const uint32_t rem = T % slots;
     g.t_full = T;
     g.sk = 1;
     g.kchunk = nk;
-    // Under-filled grid (at most half the slots): every tile is cut into K
-    // slices.  One partial round (more than half the slots, at most all):
-    // every tile is cut into the sk <= 8 slices that fill whole rounds best,
-    // within two rounds of slots, when that beats the unsplit fill by 3%
-    // (a 340-tile GEMM on 512 slots: 3 slices, 1020 items in 2 rounds, 66%
-    // -> 99.6% of the slots busy).  In every case nsplit * sk <= 2 * slots,
-    // and every slice is non-empty.
     uint32_t nsplit = 0, sk = 0;
-    if (T > slots && rem > 0 && rem <= slots / 2 && nk >= 32) {
-        nsplit = rem;
-        sk = std::min<uint32_t>(slots / rem, (uint32_t)(nk / 8));
-    } else if (T <= slots / 2 && nk >= 16) {
-        nsplit = T;
-        sk = std::min<uint32_t>(slots / T, (uint32_t)(nk / 4));
-    } else if (T <= slots && nk >= 32) {
-        double best = (double)T / slots;
-        for (uint32_t c = 2; c <= 8 && nk / c >= 8 && T * c <= 2 * slots; c++) {
-            const uint32_t items = T * c;
-            const double fill = (double)items / (double)(((items + slots - 1) / slots) * slots);
-            if (fill > best * 1.03) {
-                best = fill;
-                sk = c;
-            }
-        }
-        if (sk >= 2) nsplit = T;
-    }
+    tail_split(T, slots, nk, nsplit, sk);
     if (sk >= 2) {
         const int64_t kchunk = (nk + sk - 1) / sk;
         sk = (uint32_t)((nk + kchunk - 1) / kchunk);
@@ -634,7 +701,8 @@ int launch_cfg(GemmArgs g, hipStream_t s)
 template <int BM, int BN, int WM, int WN>
 int launch_group(GemmGroupArgs ga, hipStream_t s)
 {
-    auto kern = gemm_f64_group_kernel<BM, BN, WM, WN>;
+    auto kern = gemm_f64_group_kernel<BM, BN, WM, WN, false>;
+    auto kern_tail = gemm_f64_group_kernel<BM, BN, WM, WN, true>;
     constexpr size_t lds = (size_t)2 * (BM + BN) * BK * sizeof(double);
     if constexpr (lds > 65536) {
         static std::once_flag once;
@@ -642,21 +710,66 @@ int launch_group(GemmGroupArgs ga, hipStream_t s)
         std::call_once(once, [&] {
             attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (attr_err == hipSuccess)
+                attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(kern_tail),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         });
         DTCF_HIP(attr_err);
     }
-    uint64_t wg = 0;
+    uint64_t total = 0;
+    int64_t nk_min = INT64_MAX;
     for (int i = 0; i < ga.np; i++) {
         GemmGroupProb &P = ga.p[i];
         P.tiles_m = (uint32_t)((P.m + BM - 1) / BM);
         P.tiles_n = (uint32_t)((P.n + BN - 1) / BN);
-        P.wg0 = (uint32_t)wg;
-        wg += ((uint64_t)P.tiles_m * P.tiles_n + 7) & ~7ull;
+        total += (uint64_t)P.tiles_m * P.tiles_n;
+        nk_min = std::min<int64_t>(nk_min, (P.k + BK - 1) / BK);
     }
-    DTCF_REQUIRE(wg < (1ull << 31), "gemm group: too many tiles");
-    if (wg == 0) return DTCF_OK;
-    hipLaunchKernelGGL(kern, dim3((unsigned)wg), dim3(NT), lds, s, ga);
+    DTCF_REQUIRE(total + 8 * kGroupMax < (1ull << 31), "gemm group: too many tiles");
+    if (total == 0) return DTCF_OK;
+    // Tail split over the whole group: the last tiles (of the last
+    // problems, the shortest in K) are cut into K slices as a single GEMM's
+    // would be, with the shortest K of the group bounding the slice count.
+    int cus = 0;
+    int st = device_cus(&cus);
+    if (st != DTCF_OK) return st;
+    uint32_t nsplit = 0, sk = 0;
+    tail_split((uint32_t)total, (uint32_t)cus * (uint32_t)Occ<BM, BN>::v, nk_min, nsplit, sk);
+    if (sk < 2) nsplit = 0;
+    ga.sk = sk < 2 ? 1 : sk;
+    ga.ws = nullptr;
+    if (nsplit) {
+        st = split_workspace(s, (size_t)nsplit * sk * BM * BN * sizeof(double), &ga.ws);
+        if (st != DTCF_OK) return st;
+    }
+    uint32_t left = nsplit;
+    for (int i = ga.np - 1; i >= 0; i--) {
+        GemmGroupProb &P = ga.p[i];
+        const uint32_t T = P.tiles_m * P.tiles_n, take = std::min(left, T);
+        P.t_whole = T - take;
+        left -= take;
+        const int64_t nk = (P.k + BK - 1) / BK;
+        P.kchunk = (nk + ga.sk - 1) / ga.sk;
+    }
+    uint64_t wg = 0;
+    uint32_t tail = 0;
+    for (int i = 0; i < ga.np; i++) {
+        GemmGroupProb &P = ga.p[i];
+        P.wg0 = (uint32_t)wg;
+        wg += ((uint64_t)P.t_whole + 7) & ~7ull;
+        P.tail0 = tail;
+        tail += P.tiles_m * P.tiles_n - P.t_whole;
+    }
+    ga.wg_whole = (uint32_t)wg;
+    const uint64_t grid = (uint64_t)ga.wg_whole + (uint64_t)nsplit * ga.sk;
+    if (nsplit) hipLaunchKernelGGL(kern_tail, dim3((unsigned)grid), dim3(NT), lds, s, ga);
+    else hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, s, ga);
     DTCF_HIP(hipGetLastError());
+    if (nsplit) {
+        hipLaunchKernelGGL((group_split_reduce_kernel<BM, BN>), dim3((BM * BN + 1023) / 1024, nsplit),
+                           dim3(256), 0, s, ga);
+        DTCF_HIP(hipGetLastError());
+    }
     return DTCF_OK;
 }
 

@@ -54,7 +54,10 @@ int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
 
 // Up to kGroupMax independent dense GEMMs C_p (+)= alpha * A_p * B_p^T in
 // one launch (A_p m x k, B_p n x k, both k-contiguous; C_p row-major).
-// tiles_m, tiles_n and wg0 are filled in by the launch.
+// tiles_m, tiles_n, wg0, t_whole, tail0 and kchunk are filled in by the
+// launch: problem p's tiles [0, t_whole) run whole, its tiles [t_whole,
+// tiles) are tail tiles tail0.. of the launch, each cut into `sk` K slices
+// of kchunk steps (the tail split of a single GEMM, over the group).
 constexpr int kGroupMax = 16;
 struct GemmGroupProb {
     int64_t m, n, k;
@@ -64,11 +67,15 @@ struct GemmGroupProb {
     int64_t lda, ldb, ldc;
     int32_t vec2a, vec2b;
     uint32_t tiles_m, tiles_n, wg0;
+    uint32_t t_whole, tail0;
+    int64_t kchunk;
 };
 struct GemmGroupArgs {
     double alpha;
     int32_t overwrite;
     int32_t np;
+    uint32_t wg_whole, sk;  // workgroups of whole tiles; K slices per tail tile
+    double *ws;             // tail slices, [tail tile][slice][BM][BN]
     GemmGroupProb p[kGroupMax];
 };
 

@@ -175,6 +175,42 @@ def test_gemm_group_f64_vs_oracle(dev, cfg):
             assert np.array_equal(got.reshape(m, ldc)[:, n:], ref.reshape(m, ldc)[:, n:])
 
 
+@pytest.mark.parametrize("cfg", [0, 3])
+@pytest.mark.parametrize("case", ["tail", "underfilled"])
+def test_gemm_group_tail_split(dev, cfg, case):
+    """Grouped launches whose last tiles are cut into K slices: 'tail' has
+    549 (128x128) / 2188 (64x64) tiles on 512 slots, so the last partial
+    round's tiles (all of the last problem's and some of the one before it)
+    are split; 'underfilled' has fewer tiles than half the slots, so every
+    tile is split.  Accumulate and overwrite, ragged edges, against numpy."""
+    import ctypes as C
+
+    from dtcf_amd import blocksparse as bsp
+    from dtcf_amd._lib import check, lib
+
+    shapes = ([(2000, 2000, 520), (600, 600, 1040), (2000, 2000, 520), (300, 500, 520)]
+              if case == "tail" else [(500, 500, 600), (450, 510, 640), (500, 300, 600), (130, 70, 520)])
+    rng = np.random.default_rng(77 + cfg)
+    for overwrite in (0, 1):
+        probs, keep, refs = [], [], []
+        for m, n, k in shapes:
+            A = rng.standard_normal((m, k))
+            B = rng.standard_normal((n, k))
+            C0 = rng.standard_normal((m, n))
+            Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+            Cd = torch.from_numpy(C0.copy()).to(dev)
+            keep += [Ad, Bd, Cd]
+            probs.append(bsp.GemmProblem(m, n, k, Ad.data_ptr(), k, Bd.data_ptr(), k, Cd.data_ptr(), n))
+            refs.append((Cd, (0 if overwrite else C0) + 0.75 * (A @ B.T)))
+        arr = (bsp.GemmProblem * len(probs))(*probs)
+        say("gemm group split", case, cfg, overwrite)
+        check(lib().dtcf_gemm_group_f64(C.cast(arr, C.c_void_p), len(probs), 0.75, overwrite, cfg,
+                                        C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        torch.cuda.synchronize()
+        for i, (Cd, ref) in enumerate(refs):
+            assert relerr(Cd.cpu().numpy(), ref) <= GEMM_TOL, (case, cfg, overwrite, i)
+
+
 @pytest.mark.parametrize("m,n,k", [(2200, 2100, 520), (2049, 2300, 1040)])
 def test_gemm_one_partial_round_split(dev, m, n, k):
     """More tiles than half the CU slots but at most all of them: every tile
