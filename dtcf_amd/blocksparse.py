@@ -64,7 +64,9 @@ def _declare():
     L.dtcf_blockmove_scatter_f64.argtypes = [vp, C.c_double, vp, vp, vp]
     L.dtcf_blockmove_destroy.argtypes = [vp]
     L.dtcf_gemm_group_f64.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int, vp]
-    for f in ("dtcf_gemm_group_f64", "dtcf_grouped_plan", "dtcf_grouped_info", "dtcf_grouped_run", "dtcf_grouped_destroy",
+    L.dtcf_gemm_group_ex_f64.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                         C.c_int, vp]
+    for f in ("dtcf_gemm_group_f64", "dtcf_gemm_group_ex_f64", "dtcf_grouped_plan", "dtcf_grouped_info", "dtcf_grouped_run", "dtcf_grouped_destroy",
               "dtcf_blockmove_plan", "dtcf_blockmove_info", "dtcf_blockmove_gather_f64",
               "dtcf_blockmove_scatter_f64", "dtcf_blockmove_destroy"):
         getattr(L, f).restype = C.c_int
@@ -229,9 +231,12 @@ class IrrepDense:
     over the blocks of class h laid side by side.  A and B are gathered into
     the A_h / B_h matrices (one launch each), one grouped launch computes
     every C_h (FP64 MFMA, overwriting), and C_h is added back into C's blocks
-    (one launch).  The counted flops are exactly those of the block pairs;
-    the GEMMs also run over K_h's zero padding to a multiple of 16 (< 1%
-    extra at these sizes)."""
+    (one launch).  An operand whose fastest (last) index is external is
+    gathered as its transpose, K_h x M_h (or K_h x N_h), so the gather runs
+    along the blocks' contiguous runs, and the GEMM reads it in that layout
+    (a B gather through transposed tiles ran 1.8-2.0 TB/s).  The counted
+    flops are exactly those of the block pairs; the GEMMs also run over K_h's
+    zero padding to a multiple of 16 (< 1% extra at these sizes)."""
 
     def __init__(self, term):
         t = term
@@ -252,7 +257,10 @@ class IrrepDense:
         aops, abase, apitch = [], [], []
         bops, bbase, bpitch = [], [], []
         cops, cbase, cpitch = [], [], []
-        self.gemms = []  # (m, n, padded k, a_off, b_off, c_off)
+        # operands whose fastest index is external: gathered transposed
+        self.transa = int(t.a_idx[-1] in a_ext)
+        self.transb = int(t.b_idx[-1] in b_ext)
+        self.gemms = []  # (m, n, padded k, a_off, b_off, c_off, lda, ldb)
         self.flops = 0.0
         la = lb = lc = 0
         for h in range(NIRREP):
@@ -266,42 +274,55 @@ class IrrepDense:
             # K padded to the GEMM's K step with zero columns (never written
             # by the gathers), so every tile takes the unchecked 16-byte loads
             Kp = -(-K // 16) * 16
+            # transposed layouts: K_h rows (padding rows zero) of an even pitch
+            lda = M + M % 2 if self.transa else Kp
+            ldb = N + N % 2 if self.transb else Kp
             roff = np.concatenate([[0], np.cumsum(rs)])
             coff = np.concatenate([[0], np.cumsum(cs)])
             koff = np.concatenate([[0], np.cumsum(ks)])
             for i, gr in enumerate(R):
                 for k, gk in enumerate(Kc):
                     ga = assemble(t.a_idx, a_ext, gr, contr, gk)
-                    aops.append(block_op(t.A, t.a_idx, ga, a_ext, contr))
-                    abase.append(la + int(roff[i]) * Kp + int(koff[k]))
-                    apitch.append(Kp)
+                    if self.transa:
+                        aops.append(block_op(t.A, t.a_idx, ga, contr, a_ext))
+                        abase.append(la + int(koff[k]) * lda + int(roff[i]))
+                    else:
+                        aops.append(block_op(t.A, t.a_idx, ga, a_ext, contr))
+                        abase.append(la + int(roff[i]) * lda + int(koff[k]))
+                    apitch.append(lda)
             for j, gc in enumerate(Cc):
                 for k, gk in enumerate(Kc):
                     gb = assemble(t.b_idx, b_ext, gc, contr, gk)
-                    bops.append(block_op(t.B, t.b_idx, gb, b_ext, contr))
-                    bbase.append(lb + int(coff[j]) * Kp + int(koff[k]))
-                    bpitch.append(Kp)
+                    if self.transb:
+                        bops.append(block_op(t.B, t.b_idx, gb, contr, b_ext))
+                        bbase.append(lb + int(koff[k]) * ldb + int(coff[j]))
+                    else:
+                        bops.append(block_op(t.B, t.b_idx, gb, b_ext, contr))
+                        bbase.append(lb + int(coff[j]) * ldb + int(koff[k]))
+                    bpitch.append(ldb)
             for i, gr in enumerate(R):
                 for j, gc in enumerate(Cc):
                     gcb = assemble(t.c_idx, a_ext, gr, b_ext, gc)
                     cops.append(block_op(t.C, t.c_idx, gcb, a_ext, b_ext))
                     cbase.append(lc + int(roff[i]) * N + int(coff[j]))
                     cpitch.append(N)
-            self.gemms.append((M, N, Kp, la, lb, lc))
+            self.gemms.append((M, N, Kp, la, lb, lc, lda, ldb))
             self.flops += 2.0 * M * N * K
-            la += M * Kp
-            lb += N * Kp
+            la += (Kp if self.transa else M) * lda
+            lb += (Kp if self.transb else N) * ldb
             lc += M * N
         self.mA, self.mB, self.mC = (BlockMove(aops, abase, apitch), BlockMove(bops, bbase, bpitch),
                                      BlockMove(cops, cbase, cpitch))
+        # every move stays inside the dense buffers allocated below
+        assert self.mA.span <= max(1, la) and self.mB.span <= max(1, lb) and self.mC.span <= max(1, lc)
         dev = t.C.data.device
         self.dA = torch.zeros(max(1, la), dtype=torch.float64, device=dev)
         self.dB = torch.zeros(max(1, lb), dtype=torch.float64, device=dev)
         self.dC = torch.empty(max(1, lc), dtype=torch.float64, device=dev)
         a, b, c = self.dA.data_ptr(), self.dB.data_ptr(), self.dC.data_ptr()
         self.probs = (GemmProblem * len(self.gemms))(*[
-            GemmProblem(m, n, k, a + 8 * ao, k, b + 8 * bo, k, c + 8 * co, n)
-            for m, n, k, ao, bo, co in self.gemms])
+            GemmProblem(m, n, k, a + 8 * ao, lda, b + 8 * bo, ldb, c + 8 * co, n)
+            for m, n, k, ao, bo, co, lda, ldb in self.gemms])
 
     # C tiles of the grouped launch tried on the first run (128x128, 64x64):
     # the wider tile reuses more per load, the narrower one fills the last
@@ -309,8 +330,9 @@ class IrrepDense:
     GEMM_CFGS = (0, 3)
 
     def _gemms(self, cfg):
-        check(lib().dtcf_gemm_group_f64(C.cast(self.probs, C.c_void_p), len(self.gemms), 1.0, 1,
-                                        cfg, _s()))
+        # op(A_h) = A_h or (K x M)^T; op(B_h) = B_h^T or the K x N gather itself
+        check(lib().dtcf_gemm_group_ex_f64(C.cast(self.probs, C.c_void_p), len(self.gemms),
+                                           self.transa, 1 - self.transb, 1.0, 1, cfg, _s()))
 
     def _tune(self):
         """Times each candidate tile on the gathered operands (C_h is
@@ -346,11 +368,13 @@ class IrrepDense:
         self.nslice = ns
         dev = self.dA.device
         mx = []
-        for m, n, k, ao, bo, co in self.gemms:
+        for m, n, k, ao, bo, co, lda, ldb in self.gemms:
             ent = {}
-            for tag, rows, base in (("a", m, ao), ("b", n, bo)):
-                view = ops.View(rows, k, base + np.arange(rows, dtype=np.int64) * k,
-                                np.arange(k, dtype=np.int64))
+            for tag, rows, base, ld, tr in (("a", m, ao, lda, self.transa),
+                                            ("b", n, bo, ldb, self.transb)):
+                r, c = np.arange(rows, dtype=np.int64), np.arange(k, dtype=np.int64)
+                view = (ops.View(rows, k, base + r, c * ld) if tr else
+                        ops.View(rows, k, base + r * ld, c))
                 nb = lib().dtcf_split_bytes(prec, ns, rows, k)
                 buf = torch.empty(nb, dtype=torch.uint8, device=dev)
                 rexp = torch.zeros((rows + 127) // 128 * 128, dtype=torch.int32, device=dev)
@@ -367,7 +391,7 @@ class IrrepDense:
     def _mixed_gemms(self):
         self.dC.zero_()  # the slice products accumulate
         c0 = self.dC.data_ptr()
-        for (m, n, k, ao, bo, co), ent in zip(self.gemms, self._mx):
+        for (m, n, k, ao, bo, co, _, _), ent in zip(self.gemms, self._mx):
             _, ab, ae = ent["a"]
             _, bb, be = ent["b"]
             check(lib().dtcf_gemm_split_f64(self.prec, self.nslice, m, n, k, 1.0,

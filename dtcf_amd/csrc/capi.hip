@@ -112,17 +112,20 @@ int dtcf_gemm_f64(int transa, int transb, int64_t m, int64_t n, int64_t k, doubl
     return gemm_f64_launch(g, pick_config(m, n, k), as_stream(stream));
 }
 
-int dtcf_gemm_group_f64(const dtcf_gemm_problem *probs, int nprob, double alpha, int overwrite,
-                        int cfg, void *stream)
+int dtcf_gemm_group_ex_f64(const dtcf_gemm_problem *probs, int nprob, int transa, int transb,
+                           double alpha, int overwrite, int cfg, void *stream)
 {
     DTCF_REQUIRE(nprob >= 0 && (probs || nprob == 0), "gemm group: null problem list");
     DTCF_REQUIRE(cfg >= 0 && cfg < kCfgCount, "gemm group: unknown tile configuration");
+    DTCF_REQUIRE((transa == 0 || transa == 1) && (transb == 0 || transb == 1),
+                 "gemm group: transa / transb must be 0 or 1");
     std::vector<int> order(nprob);
     for (int i = 0; i < nprob; i++) {
         const dtcf_gemm_problem &q = probs[i];
         DTCF_REQUIRE(q.m > 0 && q.n > 0 && q.k > 0, "gemm group: empty problem");
         DTCF_REQUIRE(q.A && q.B && q.C, "gemm group: null operand");
-        DTCF_REQUIRE(q.lda >= q.k && q.ldb >= q.k && q.ldc >= q.n, "gemm group: pitch too small");
+        DTCF_REQUIRE(q.lda >= (transa ? q.m : q.k) && q.ldb >= (transb ? q.k : q.n) && q.ldc >= q.n,
+                     "gemm group: pitch too small");
         order[i] = i;
     }
     // Longest tiles first: the hardware dispatches workgroups in id order.
@@ -148,10 +151,17 @@ int dtcf_gemm_group_f64(const dtcf_gemm_problem *probs, int nprob, double alpha,
             P.vec2a = ((reinterpret_cast<uintptr_t>(q.A) & 15) == 0) && (q.lda % 2 == 0);
             P.vec2b = ((reinterpret_cast<uintptr_t>(q.B) & 15) == 0) && (q.ldb % 2 == 0);
         }
-        const int st = gemm_f64_group_launch(ga, cfg, as_stream(stream));
+        const int st = gemm_f64_group_launch(ga, cfg, transa ? kDMN : kDK, transb ? kDK : kDMN,
+                                             as_stream(stream));
         if (st != DTCF_OK) return st;
     }
     return DTCF_OK;
+}
+
+int dtcf_gemm_group_f64(const dtcf_gemm_problem *probs, int nprob, double alpha, int overwrite,
+                        int cfg, void *stream)
+{
+    return dtcf_gemm_group_ex_f64(probs, nprob, 0, 1, alpha, overwrite, cfg, stream);
 }
 
 int dtcf_view_create(int64_t rows, int64_t cols, const int64_t *rofs, const int64_t *cofs,

@@ -446,7 +446,7 @@ __device__ inline int tail_problem(const GemmGroupArgs &ga, uint32_t ti)
 // each own one K slice of a tail tile (TAIL: a launch without tail tiles
 // compiles the slice path out, which keeps the 192 x 96 tiles' register
 // spills at their level without it).
-template <int BM, int BN, int WM, int WN, bool TAIL>
+template <int BM, int BN, int WM, int WN, bool TAIL, int AM, int BMD>
 __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_group_kernel(GemmGroupArgs ga)
 {
     const uint32_t bid = blockIdx.x;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(NT, (Occ<BM, BN>::v)) void gemm_f64_group_kernel(Ge
     g.tiles_n = P.tiles_n;
     g.overwrite = ga.overwrite;
     g.ws = ga.ws;
-    gemm_tile<BM, BN, WM, WN, kDK, kDK, false>(g, q, kt0, kt1, TAIL && split, sidx);
+    gemm_tile<BM, BN, WM, WN, AM, BMD, false>(g, q, kt0, kt1, TAIL && split, sidx);
 }
 
 // Sums the K slices of each tail tile of a group in slice order and adds
@@ -698,11 +698,11 @@ int launch_cfg(GemmArgs g, hipStream_t s)
     return DTCF_ERR_INVALID;
 }
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int AM, int BMD>
 int launch_group(GemmGroupArgs ga, hipStream_t s)
 {
-    auto kern = gemm_f64_group_kernel<BM, BN, WM, WN, false>;
-    auto kern_tail = gemm_f64_group_kernel<BM, BN, WM, WN, true>;
+    auto kern = gemm_f64_group_kernel<BM, BN, WM, WN, false, AM, BMD>;
+    auto kern_tail = gemm_f64_group_kernel<BM, BN, WM, WN, true, AM, BMD>;
     constexpr size_t lds = (size_t)2 * (BM + BN) * BK * sizeof(double);
     if constexpr (lds > 65536) {
         static std::once_flag once;
@@ -773,19 +773,30 @@ int launch_group(GemmGroupArgs ga, hipStream_t s)
     return DTCF_OK;
 }
 
+template <int BM, int BN, int WM, int WN>
+int launch_group_modes(const GemmGroupArgs &ga, int amode, int bmode, hipStream_t s)
+{
+    if (amode == kDK && bmode == kDK) return launch_group<BM, BN, WM, WN, kDK, kDK>(ga, s);
+    if (amode == kDK && bmode == kDMN) return launch_group<BM, BN, WM, WN, kDK, kDMN>(ga, s);
+    if (amode == kDMN && bmode == kDK) return launch_group<BM, BN, WM, WN, kDMN, kDK>(ga, s);
+    if (amode == kDMN && bmode == kDMN) return launch_group<BM, BN, WM, WN, kDMN, kDMN>(ga, s);
+    set_error("gemm group: unsupported operand layout");
+    return DTCF_ERR_INVALID;
+}
+
 }  // namespace
 
-int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, hipStream_t s)
+int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, int amode, int bmode, hipStream_t s)
 {
     DTCF_REQUIRE(ga.np >= 0 && ga.np <= kGroupMax, "gemm group: too many problems");
     for (int i = 0; i < ga.np; i++)
         DTCF_REQUIRE(ga.p[i].m > 0 && ga.p[i].n > 0 && ga.p[i].k > 0, "gemm group: empty problem");
     if (ga.np == 0) return DTCF_OK;
     switch (cfg) {
-    case kCfg128x128: return launch_group<128, 128, 64, 64>(ga, s);
-    case kCfg192x96: return launch_group<192, 96, 96, 48>(ga, s);
-    case kCfg96x192: return launch_group<96, 192, 48, 96>(ga, s);
-    case kCfg64x64: return launch_group<64, 64, 32, 32>(ga, s);
+    case kCfg128x128: return launch_group_modes<128, 128, 64, 64>(ga, amode, bmode, s);
+    case kCfg192x96: return launch_group_modes<192, 96, 96, 48>(ga, amode, bmode, s);
+    case kCfg96x192: return launch_group_modes<96, 192, 48, 96>(ga, amode, bmode, s);
+    case kCfg64x64: return launch_group_modes<64, 64, 32, 32>(ga, amode, bmode, s);
     default: break;
     }
     set_error("gemm group: unknown tile configuration");

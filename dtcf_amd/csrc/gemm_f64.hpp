@@ -52,8 +52,9 @@ struct GemmArgs {
 
 int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s);
 
-// Up to kGroupMax independent dense GEMMs C_p (+)= alpha * A_p * B_p^T in
-// one launch (A_p m x k, B_p n x k, both k-contiguous; C_p row-major).
+// Up to kGroupMax independent dense GEMMs C_p (+)= alpha * op(A_p) * op(B_p)
+// in one launch, every problem with the same operand layouts: A_p m x k
+// (kDK) or k x m (kDMN), B_p n x k (kDK) or k x n (kDMN); C_p row-major.
 // tiles_m, tiles_n, wg0, t_whole, tail0 and kchunk are filled in by the
 // launch: problem p's tiles [0, t_whole) run whole, its tiles [t_whole,
 // tiles) are tail tiles tail0.. of the launch, each cut into `sk` K slices
@@ -79,7 +80,7 @@ struct GemmGroupArgs {
     GemmGroupProb p[kGroupMax];
 };
 
-int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, hipStream_t s);
+int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, int amode, int bmode, hipStream_t s);
 
 // Allocates the tail-split workspace of stream `s` at the largest size any
 // GEMM on it can need, so later launches on `s` never allocate.

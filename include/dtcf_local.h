@@ -354,6 +354,15 @@ typedef struct {
 } dtcf_gemm_problem;
 int dtcf_gemm_group_f64(const dtcf_gemm_problem *probs, int nprob, double alpha, int overwrite,
                         int cfg, void *stream);
+/* The same with the operand layouts of dtcf_gemm_f64, one for the whole
+ * group: C_p (+)= alpha * op(A_p) * op(B_p), op(A) = A [m x k] (transa = 0,
+ * lda >= k) or A^T with A [k x m] (transa = 1, lda >= m); op(B) = B^T with
+ * B [n x k] (transb = 1, ldb >= k) or B [k x n] (transb = 0, ldb >= n).
+ * dtcf_gemm_group_f64 is transa = 0, transb = 1.  A block gathered from a
+ * tensor whose fastest index is external lands in the k x n layout along
+ * its contiguous runs (no transposed gather). */
+int dtcf_gemm_group_ex_f64(const dtcf_gemm_problem *probs, int nprob, int transa, int transb,
+                           double alpha, int overwrite, int cfg, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* 6. Mixed precision with FP64 recombination (new; the reference is    */

@@ -102,6 +102,15 @@ def test_gemm_group_rejects_bad_problems_without_gpu_calls():
     arr = (P * 1)(ok)
     assert L.dtcf_gemm_group_f64(C.cast(arr, C.c_void_p), 1, 1.0, 1, 9, None) == -1
     assert L.dtcf_gemm_group_f64(None, 0, 1.0, 1, 0, None) == 0
+    # the layout form checks pitches against its layouts: A k x m needs lda >= m
+    ex = L.dtcf_gemm_group_ex_f64
+    tall = (P * 1)(P(8, 4, 4, 16, 4, 16, 4, 16, 4))
+    assert ex(C.cast(tall, C.c_void_p), 1, 1, 1, 1.0, 1, 0, None) == -1
+    assert b"pitch" in L.dtcf_last_error()
+    wide_b = (P * 1)(P(4, 8, 4, 16, 4, 16, 4, 16, 8))   # B k x n needs ldb >= n
+    assert ex(C.cast(wide_b, C.c_void_p), 1, 0, 0, 1.0, 1, 0, None) == -1
+    assert ex(C.cast(arr, C.c_void_p), 1, 2, 1, 1.0, 1, 0, None) == -1
+    assert b"transa" in L.dtcf_last_error()
 
 
 def test_gemm_split_level_one_is_empty_without_gpu_calls():

@@ -39,6 +39,9 @@ TERMS = [
     ("vvoo", "abij", "vvvv", "abcd", "vvoo", "cdij"),    # ladder (config 1 shape)
     ("vvoo", "abij", "vv", "ac", "vvoo", "cbij"),        # Fock-like (2-D x 4-D)
     ("oo", "ij", "oovv", "ikab", "oovv", "jkab"),        # T1/intermediate-like
+    # fastest index external: A (then A and B) gathered transposed in cfg 4
+    ("oovv", "ijab", "ovov", "kcjb", "oovv", "ikac"),
+    ("oovv", "ijab", "ovov", "kcjb", "ovov", "kcia"),
 ]
 
 

@@ -211,6 +211,51 @@ def test_gemm_group_tail_split(dev, cfg, case):
             assert relerr(Cd.cpu().numpy(), ref) <= GEMM_TOL, (case, cfg, overwrite, i)
 
 
+@pytest.mark.parametrize("cfg", [0, 3])
+@pytest.mark.parametrize("transa,transb", [(0, 1), (1, 1), (0, 0), (1, 0)])
+def test_gemm_group_ex_layouts(dev, transa, transb, cfg):
+    """dtcf_gemm_group_ex_f64: A as m x k or k x m, B as n x k or k x n, odd
+    pitches and extents, overwrite and accumulate; the first group has 549
+    / 2188 tiles (tail split), the second a few small problems."""
+    import ctypes as C
+
+    from dtcf_amd import blocksparse as bsp
+    from dtcf_amd._lib import check, lib
+
+    rng = np.random.default_rng(5 + 2 * transa + transb + cfg)
+    groups = [[(2000, 2000, 520), (600, 600, 1040), (2000, 2000, 520), (300, 500, 520)],
+              [(17, 33, 5), (130, 71, 257), (201, 150, 31), (64, 64, 16), (1, 9, 3)]]
+    for gi, shapes in enumerate(groups):
+        for overwrite in (0, 1):
+            probs, keep, refs = [], [], []
+            for i, (m, n, k) in enumerate(shapes):
+                A = rng.standard_normal((k, m) if transa else (m, k))
+                B = rng.standard_normal((n, k) if transb else (k, n))
+                lda = A.shape[1] + i % 2
+                ldb = B.shape[1] + (i + 1) % 2
+                Ap = np.zeros((A.shape[0], lda))
+                Ap[:, :A.shape[1]] = A
+                Bp = np.zeros((B.shape[0], ldb))
+                Bp[:, :B.shape[1]] = B
+                C0 = rng.standard_normal((m, n))
+                Ad, Bd = torch.from_numpy(Ap).to(dev), torch.from_numpy(Bp).to(dev)
+                Cd = torch.from_numpy(C0.copy()).to(dev)
+                keep += [Ad, Bd, Cd]
+                probs.append(bsp.GemmProblem(m, n, k, Ad.data_ptr(), lda, Bd.data_ptr(), ldb,
+                                             Cd.data_ptr(), n))
+                opa = A.T if transa else A
+                opb = B.T if transb else B
+                refs.append((Cd, (0 if overwrite else C0) - 1.5 * (opa @ opb)))
+            arr = (bsp.GemmProblem * len(probs))(*probs)
+            say("gemm group ex", transa, transb, cfg, gi, overwrite)
+            check(lib().dtcf_gemm_group_ex_f64(C.cast(arr, C.c_void_p), len(probs), transa, transb,
+                                               -1.5, overwrite, cfg,
+                                               C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+            torch.cuda.synchronize()
+            for i, (Cd, ref) in enumerate(refs):
+                assert relerr(Cd.cpu().numpy(), ref) <= GEMM_TOL, (gi, overwrite, i)
+
+
 @pytest.mark.parametrize("m,n,k", [(2200, 2100, 520), (2049, 2300, 1040)])
 def test_gemm_one_partial_round_split(dev, m, n, k):
     """More tiles than half the CU slots but at most all of them: every tile
