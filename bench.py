@@ -21,6 +21,7 @@ Prints ONE JSON line.  --gpus N > 1 exits: one rank's share runs on one GPU.
 from __future__ import annotations
 
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -314,6 +315,66 @@ def leg_blocksparse(dev, log, steps=5):
     return out
 
 
+def leg_symmetric(dev, log, o=48, v=192, steps=3):
+    """The C2 ring step with A symmetric in (i,k) ('aacc', vgrid 4x4x4x4):
+    the stored tiles (addr0 >= addr1) and the bounced copies of the others
+    (the stored tile's data under the swapped address), so A has two bounce
+    classes.  Times LITERAL, PACKED and FUSED, and checks each against the
+    FUSED step on the dense tiles of the same tensor."""
+    import torch
+
+    from dtcf_amd import mixed, rrr
+
+    vg = [4, 4, 4, 4]
+    br = [o // 4, o // 4, v // 4, v // 4]
+    n = o * o * v * v
+    X = mixed.fill_normal(n, 11, 0.05, dev).view(o, o, v, v)
+    full = X + X.transpose(0, 1)
+    del X
+    # dense tiles [a0][a1][a2][a3][block], row-major addresses
+    dense = full.view(4, br[0], 4, br[1], 4, br[2], 4, br[3]).permute(0, 2, 4, 6, 1, 3, 5, 7) \
+        .contiguous().view(4, 4, 4, 4, -1)
+    del full
+    addrs = np.array(list(itertools.product(range(4), repeat=4)), dtype=np.int32)
+    src = torch.tensor([[a1, a0, a2, a3] if a0 < a1 else [a0, a1, a2, a3] for a0, a1, a2, a3 in addrs],
+                       device=dev, dtype=torch.long)
+    sym_tiles = dense[src[:, 0], src[:, 1], src[:, 2], src[:, 3]].reshape(-1)
+    tA_sym = rrr.TileTensor("aacc", [o, o, v, v], vg, dev, data=sym_tiles, addrs=addrs)
+    tA_dense = rrr.TileTensor("cccc", [o, o, v, v], vg, dev, data=dense.reshape(-1), addrs=addrs)
+    tB = rrr.TileTensor("cccc", [o, o, v, v], vg, dev, data=mixed.fill_normal(n, 12, 0.05, dev),
+                        addrs=addrs)
+    C0 = mixed.fill_normal(n, 13, 0.05, dev)
+    tC = rrr.TileTensor("cccc", [o, o, v, v], vg, dev, data=C0.clone(), addrs=addrs)
+    ref = rrr.LocalStep(tA_dense, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=rrr.FUSED)
+    ref.run(1.0)
+    torch.cuda.synchronize()
+    want = tC.tiles.clone()
+    del ref
+    out = {"workload": f"ring step o={o} v={v} with A symmetric in (i,k) ('aacc'), 160 stored "
+                       f"tiles + 96 bounced copies", "modes": {}}
+    for name, mode in (("literal", rrr.LITERAL), ("packed", rrr.PACKED), ("fused", rrr.FUSED)):
+        st = rrr.LocalStep(tA_sym, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=mode)
+        tC.tiles.copy_(C0)
+        st.run(1.0)
+        torch.cuda.synchronize()
+        err = float((tC.tiles - want).abs().max() / want.abs().max())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.run(1.0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out["modes"][name] = {"ms_per_step": round(dt * 1e3, 3),
+                              "tflops": round(st.flops / dt / 1e12, 3),
+                              "max_rel_diff_vs_dense_fused": err}
+        log(f"[bench] symmetric A, {name}: {dt*1e3:.3f} ms/step, rel diff {err:.2e}")
+        del st
+        torch.cuda.synchronize()
+    del tA_sym, tA_dense, tB, tC, dense, sym_tiles, C0, want
+    torch.cuda.empty_cache()
+    return out
+
+
 def leg_c1(dev, log, reps=20):
     """C1 CCD ladder R[a,b,i,j] += 0.5 V[a,b,c,d] T[c,d,i,j] at o=8, v=32
     (ccsd-t2.cc:57-61) through the DLTC term path: latency of the whole
@@ -521,7 +582,8 @@ def main():
     torch.cuda.empty_cache()
     extra = {}
     if not args.no_extra:
-        for key, fn in (("c1", lambda: leg_c1(dev, log)),
+        for key, fn in (("symmetric", lambda: leg_symmetric(dev, log)),
+                        ("c1", lambda: leg_c1(dev, log)),
                         ("dropin", lambda: leg_dropin(log)),
                         ("ccsd", lambda: leg_ccsd(log)),
                         ("precision", lambda: leg_precision(log)),
