@@ -438,24 +438,25 @@ def test_local_step_symmetric_bounce(dev, mode):
     assert relerr(got, ref) <= GEMM_TOL
 
 
+@pytest.mark.parametrize("which", ["AB", "B"])
 @pytest.mark.parametrize("mode", [rrr.LITERAL, rrr.PACKED, rrr.FUSED])
-def test_local_step_both_operands_symmetric(dev, mode):
-    # A symmetric in (i,k), B in (k,j) ('aacc' both), uneven tile extents
-    # (o=9 in 3 tiles, v=10 in 2) and the contraction index k split over
-    # row and column roles of the two operands: both bounced operands are
-    # packed by class, C is read through its view.
+def test_local_step_both_operands_symmetric(dev, mode, which):
+    # A symmetric in (i,k), B in (k,j) ('aacc'), or B alone, uneven tile
+    # extents (o=9 in 3 tiles, v=10 in 2) and the contraction index k split
+    # over row and column roles of the two operands: the bounced operands
+    # are packed by class, the others and C read through their views.
     o, v = 9, 10
     vg = [3, 3, 2, 2]
     rng = np.random.default_rng(21)
     X = rng.standard_normal((o, o, v, v))
-    fa = X + X.transpose(1, 0, 2, 3)
+    fa = X + X.transpose(1, 0, 2, 3) if "A" in which else X
     Y = rng.standard_normal((o, o, v, v))
     fb = Y + Y.transpose(1, 0, 2, 3)
-    tA = _bounced("aacc", fa, vg, dev)
+    tA = _bounced("aacc", fa, vg, dev) if "A" in which else rrr.TileTensor.from_dense("cccc", fa, vg, dev)
     tB = _bounced("aacc", fb, vg, dev)
     Cm = rng.standard_normal((o, o, v, v))
     tC = rrr.TileTensor.from_dense("cccc", Cm, vg, dev)
-    say("both symmetric, mode", mode)
+    say("symmetric", which, "mode", mode)
     st = rrr.LocalStep(tA, "i,k,a,c", tB, "k,j,c,b", tC, "i,j,a,b", mode=mode)
     for _ in range(2):  # a second run re-packs over the first run's big matrices
         st.run(0.5)
