@@ -122,7 +122,7 @@ MIXED = [((mixed.I8, 5, 6), 5e-10), ((mixed.I8, 4, 5), 1e-7), ((mixed.I8, 3, 4),
 
 
 @pytest.mark.parametrize("mode,bound", MIXED, ids=lambda x: str(x))
-@pytest.mark.parametrize("term", TERMS[:2], ids=["ring", "ladder"])
+@pytest.mark.parametrize("term", TERMS[:2] + TERMS[5:], ids=["ring", "ladder", "both_kxn"])
 def test_mixed_dense_by_irrep_vs_fp64(dev, term, mode, bound):
     """The dense-by-irrep form in a mixed mode against its FP64 form on the
     same block-sparse operands; a second run with A resident (slices kept)
