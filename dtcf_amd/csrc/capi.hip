@@ -328,6 +328,38 @@ static int gemm_view_impl(int64_t m, int64_t n, int64_t k, double alpha, const d
     DTCF_REQUIRE(A && B && C && av && bv, "gemm_view: null operand or view");
     DTCF_REQUIRE(av->rows == m && av->cols == k, "gemm_view: A view is not m x k");
     DTCF_REQUIRE(bv->rows == n && bv->cols == k, "gemm_view: B view is not n x k");
+    if (cv) DTCF_REQUIRE(cv->rows == m && cv->cols == n, "gemm_view: C view is not m x n");
+    else DTCF_REQUIRE(ldc >= n, "gemm_view: ldc < n");
+    // A matrix-vector product (n = 1 or m = 1) whose long operand has its
+    // column pairs adjacent and 16-byte aligned runs as a streaming product
+    // (gemv_kernel) instead of MFMA tiles of one useful column.
+    {
+        const bool col = n == 1, row = !col && m == 1;
+        const dtcf_view_s *mv = col ? av : bv, *xv = col ? bv : av;
+        const double *M = col ? A : B, *X = col ? B : A;
+        if ((col || row) && mv->colpairs && !mv->blk && k % 2 == 0 && k <= kGemvMaxK &&
+            (reinterpret_cast<uintptr_t>(M) & 15) == 0) {
+            GemvArgs q{};
+            q.rows = col ? m : n;
+            q.k = k;
+            q.alpha = alpha;
+            q.M = M;
+            q.mrofs = mv->rofs;
+            q.mcofs = mv->cofs;
+            q.X = X;
+            fill_op(q.xv, xv);
+            q.Y = C;
+            q.ycol = col ? 1 : 0;
+            q.overwrite = overwrite;
+            if (cv) {
+                fill_op(q.yv, cv);
+                q.yview = 1;
+            } else {
+                q.ldy = ldc;
+            }
+            return gemv_f64_launch(q, as_stream(stream));
+        }
+    }
     GemmArgs g{};
     g.m = m;
     g.n = n;
@@ -350,12 +382,10 @@ static int gemm_view_impl(int64_t m, int64_t n, int64_t k, double alpha, const d
     else if (bv->rowpairs) { g.bmode = kVMN; g.b.vec2 = bal; }
     else { g.bmode = kVK; g.b.vec2 = 0; }
     if (cv) {
-        DTCF_REQUIRE(cv->rows == m && cv->cols == n, "gemm_view: C view is not m x n");
         fill_op(g.c, cv);
         g.cview = 1;
         return gemm_f64_launch(g, pick_config(m, n, k), as_stream(stream));
     }
-    DTCF_REQUIRE(ldc >= n, "gemm_view: ldc < n");
     g.c.ld = ldc;
     g.cview = 0;
     return gemm_f64_launch(g, pick_config(m, n, k), as_stream(stream));

@@ -538,6 +538,53 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(GemmArgs g)
     }
 }
 
+// Matrix-vector product (GemvArgs): the product is HBM-bound (every element
+// of M is used once), so instead of MFMA tiles that would run 63 of 64
+// columns on padding, each wave takes one row at a time, its lanes read
+// 16-byte column pairs 128 columns apart (eight loads in flight per lane),
+// x sits in LDS, and the wave sums its lanes in a fixed tree: the order of
+// the additions does not depend on timing.
+__global__ __launch_bounds__(256) void gemv_kernel(GemvArgs g)
+{
+    extern __shared__ double xs[];
+    for (int64_t c = threadIdx.x; c < g.k; c += 256) xs[c] = g.X[op_addr<kVK>(g.xv, 0, c)];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < g.rows; r += nw) {
+        const double *row = g.M + g.mrofs[r];
+        double acc[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[u] = 0.0;
+        int64_t c = 2 * lane;
+        for (; c + 7 * 128 < g.k; c += 8 * 128) {
+            d2 a[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] = *(const d2 *)(row + g.mcofs[c + u * 128]);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                acc[u] = fma(a[u].x, xs[c + u * 128], acc[u]);
+                acc[u] = fma(a[u].y, xs[c + u * 128 + 1], acc[u]);
+            }
+        }
+        for (int u = 0; c < g.k; c += 128, u++) {
+            const d2 a = *(const d2 *)(row + g.mcofs[c]);
+            acc[u] = fma(a.x, xs[c], acc[u]);
+            acc[u] = fma(a.y, xs[c + 1], acc[u]);
+        }
+        double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) {
+            int64_t addr;
+            if (g.yview) addr = g.ycol ? op_addr<kVK>(g.yv, r, 0) : op_addr<kVK>(g.yv, 0, r);
+            else addr = g.ycol ? r * g.ldy : r;
+            if (g.overwrite) g.Y[addr] = g.alpha * v;
+            else g.Y[addr] += g.alpha * v;
+        }
+    }
+}
+
 // CU count per device, and the tail-split workspace per (device, stream):
 // GEMMs enqueued on different streams never share a workspace; GEMMs on one
 // stream are ordered by it.
@@ -801,6 +848,24 @@ int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, int amode, int bmode
     }
     set_error("gemm group: unknown tile configuration");
     return DTCF_ERR_INVALID;
+}
+
+int gemv_f64_launch(const GemvArgs &g, hipStream_t s)
+{
+    DTCF_REQUIRE(g.k > 0 && g.k % 2 == 0 && g.k <= kGemvMaxK, "gemv: k must be even and <= 20480");
+    if (g.rows <= 0) return DTCF_OK;
+    static std::once_flag once;
+    static hipError_t attr_err = hipSuccess;
+    std::call_once(once, [&] {
+        attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(gemv_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(kGemvMaxK * sizeof(double)));
+    });
+    DTCF_HIP(attr_err);
+    const unsigned grid = (unsigned)std::min<int64_t>((g.rows + 3) / 4, 2048);
+    hipLaunchKernelGGL(gemv_kernel, dim3(grid), dim3(256), (size_t)g.k * sizeof(double), s, g);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
 }
 
 int gemm_workspace(hipStream_t s, size_t need, double **ws) { return split_workspace(s, need, ws); }

@@ -82,6 +82,26 @@ struct GemmGroupArgs {
 
 int gemm_f64_group_launch(const GemmGroupArgs &ga, int cfg, int amode, int bmode, hipStream_t s);
 
+// Matrix-vector products inside the view GEMM: y(r) (+)= alpha * sum_c
+// M(r, c) x(c) over the rows of a view M whose column pairs are adjacent and
+// 16-byte aligned (no block table), x a 1 x k view, y the single column
+// (ycol = 1: y(r) = Y(r, 0)) or single row (ycol = 0: y(r) = Y(0, r)) of a
+// dense Y (pitch ldy) or of a view.  k <= kGemvMaxK (x is staged in LDS).
+constexpr int64_t kGemvMaxK = 20480;
+struct GemvArgs {
+    int64_t rows, k;
+    double alpha;
+    const double *M;
+    const int64_t *mrofs, *mcofs;
+    const double *X;
+    OpArgs xv;
+    double *Y;
+    OpArgs yv;
+    int32_t yview, ycol, overwrite;
+    int64_t ldy;
+};
+int gemv_f64_launch(const GemvArgs &g, hipStream_t s);
+
 // Allocates the tail-split workspace of stream `s` at the largest size any
 // GEMM on it can need, so later launches on `s` never allocate.
 int gemm_reserve_workspace(hipStream_t s);

@@ -256,6 +256,57 @@ def test_gemm_group_ex_layouts(dev, transa, transb, cfg):
                 assert relerr(Cd.cpu().numpy(), ref) <= GEMM_TOL, (gi, overwrite, i)
 
 
+@pytest.mark.parametrize("orient", ["col", "row"])
+@pytest.mark.parametrize("rows,k,layout", [(5000, 9216, "pairs"), (37, 130, "pairs"), (300, 4, "pairs"),
+                                           (700, 20482, "pairs"), (900, 515, "pairs"),
+                                           (800, 256, "rows")])
+@pytest.mark.parametrize("cview", [False, True])
+def test_gemm_view_matrix_vector(dev, orient, rows, k, layout, cview):
+    """Matrix-vector products through the view GEMM (n = 1: C[:, 0] = alpha
+    A x; m = 1: C[0, :] = alpha x B^T).  The long operand's rows are
+    scrambled (a row offset table), its column pairs adjacent ('pairs': the
+    streaming path when k is even and at most 20480; k = 20482 and 515 take
+    the GEMM) or its rows adjacent ('rows': the GEMM); x is read through a
+    strided view; C is dense or a scrambled view; accumulate into C, against
+    numpy."""
+    rng = np.random.default_rng(rows + k)
+    Mh = rng.standard_normal((rows, k))
+    xh = rng.standard_normal(k)
+    perm = rng.permutation(rows)
+    if layout == "pairs":        # row r of the view is stored row perm[r], row-major
+        Mbuf = Mh[np.argsort(perm)].copy()
+        mv = ops.View(rows, k, perm.astype(np.int64) * k, np.arange(k, dtype=np.int64))
+    else:                         # column-major storage: rows adjacent
+        Mbuf = Mh.T.copy()
+        mv = ops.View(rows, k, np.arange(rows, dtype=np.int64), np.arange(k, dtype=np.int64) * rows)
+    xbuf = np.zeros(3 * k)
+    xbuf[::3] = xh                # x(c) at 3c
+    xv = ops.View(1, k, np.zeros(1, dtype=np.int64), 3 * np.arange(k, dtype=np.int64))
+    Md, xd = torch.from_numpy(Mbuf.ravel()).to(dev), torch.from_numpy(xbuf).to(dev)
+    C0 = rng.standard_normal(rows)
+    if orient == "col":           # C rows x 1
+        m, n, A, av, B, bv = rows, 1, Md, mv, xd, xv
+    else:                         # C 1 x rows
+        m, n, A, av, B, bv = 1, rows, xd, xv, Md, mv
+    cperm = rng.permutation(rows)
+    if cview:
+        Cbuf = np.zeros(rows)
+        Cbuf[cperm] = C0
+        cv = (ops.View(rows, 1, cperm.astype(np.int64), np.zeros(1, dtype=np.int64)) if orient == "col"
+              else ops.View(1, rows, np.zeros(1, dtype=np.int64), cperm.astype(np.int64)))
+    else:
+        Cbuf, cv = C0.copy(), None
+    Cd = torch.from_numpy(Cbuf).to(dev)
+    say("gemv", orient, rows, k, layout, cview)
+    ops.gemm_view(m, n, k, -0.5, A, av, B, bv, Cd, cv, n)
+    torch.cuda.synchronize()
+    got = Cd.cpu().numpy()
+    if cview:
+        got = got[cperm]
+    ref = C0 - 0.5 * (Mh @ xh)
+    assert relerr(got, ref) <= GEMM_TOL
+
+
 @pytest.mark.parametrize("m,n,k", [(2200, 2100, 520), (2049, 2300, 1040)])
 def test_gemm_one_partial_round_split(dev, m, n, k):
     """More tiles than half the CU slots but at most all of them: every tile
