@@ -307,6 +307,74 @@ def test_gemm_view_matrix_vector(dev, orient, rows, k, layout, cview):
     assert relerr(got, ref) <= GEMM_TOL
 
 
+def _random_layout(rng, rows, cols, dev):
+    """A rows x cols operand stored in a buffer with spare room, in one of
+    four layouts: row-major, column-major, row-major with scrambled rows, or
+    element offsets with a stride of 3 (no adjacent pairs).  Returns (dense
+    host matrix, device buffer, View)."""
+    M = rng.standard_normal((rows, cols))
+    kind = rng.integers(4)
+    if kind == 0:
+        buf, ro, co = M.ravel().copy(), np.arange(rows) * cols, np.arange(cols)
+    elif kind == 1:
+        buf, ro, co = M.T.ravel().copy(), np.arange(rows), np.arange(cols) * rows
+    elif kind == 2:
+        perm = rng.permutation(rows)
+        buf = M[np.argsort(perm)].ravel().copy()
+        ro, co = perm * cols, np.arange(cols)
+    else:
+        buf = np.zeros(3 * rows * cols)
+        ro, co = np.arange(rows) * 3 * cols, np.arange(cols) * 3
+        for r in range(rows):
+            buf[ro[r] + co] = M[r]
+    return M, torch.from_numpy(buf).to(dev), ops.View(rows, cols, ro.astype(np.int64), co.astype(np.int64))
+
+
+def test_gemm_view_random_shapes(dev):
+    """60 seeded random view GEMMs C (+)= alpha A B^T, m, n, k from 1 to
+    700 (ones included, so matrix-vector and outer products), every operand
+    in a random layout, C dense (with a pitch) or a scrambled view, under
+    the automatic tile choice (64 x 64 or 128 x 128, split-K in its three
+    forms, the matrix-vector path), against numpy."""
+    rng = np.random.default_rng(2024)
+    sizes = [1, 2, 3, 17, 64, 65, 127, 128, 130, 255, 256, 700]
+    for case in range(60):
+        m, n, k = (int(rng.choice(sizes)) for _ in range(3))
+        if case % 5 == 0:
+            k = int(rng.integers(1025, 3000))      # longer K: 128 x 128 tiles, K splits
+        Ah, Ad, av = _random_layout(rng, m, k, dev)
+        Bh, Bd, bv = _random_layout(rng, n, k, dev)
+        C0 = rng.standard_normal((m, n))
+        alpha = float(rng.choice([1.0, -0.75, 2.5]))
+        if rng.integers(2):
+            perm = rng.permutation(m * n)
+            Cbuf = np.zeros(m * n)
+            Cbuf[perm] = C0.ravel()
+            cv = ops.View(m, n, (perm.reshape(m, n)[:, 0]).astype(np.int64),
+                          np.zeros(n, dtype=np.int64)) if n == 1 else None
+            if cv is None:     # a scrambled-row view of a row-major C
+                rp = rng.permutation(m)
+                Cbuf = C0[np.argsort(rp)].ravel().copy()
+                cv = ops.View(m, n, (rp * n).astype(np.int64), np.arange(n, dtype=np.int64))
+                unscr = lambda x, rp=rp, m=m, n=n: x.reshape(m, n)[rp]
+            else:
+                unscr = lambda x, perm=perm, m=m, n=n: x[perm].reshape(m, n)
+            ldc = n
+        else:
+            ldc = n + int(rng.integers(0, 3))
+            Cbuf = np.zeros(m * ldc)
+            Cbuf.reshape(m, ldc)[:, :n] = C0
+            cv = None
+            unscr = lambda x, m=m, n=n, ldc=ldc: x.reshape(m, ldc)[:, :n]
+        Cd = torch.from_numpy(Cbuf).to(dev)
+        say("random gemm", case, m, n, k, cv is not None)
+        ops.gemm_view(m, n, k, alpha, Ad, av, Bd, bv, Cd, cv, ldc)
+        torch.cuda.synchronize()
+        got = unscr(Cd.cpu().numpy())
+        ref = C0 + alpha * (Ah @ Bh.T)
+        assert relerr(got, ref) <= GEMM_TOL, (case, m, n, k)
+
+
 @pytest.mark.parametrize("m,n,k", [(2200, 2100, 520), (2049, 2300, 1040)])
 def test_gemm_one_partial_round_split(dev, m, n, k):
     """More tiles than half the CU slots but at most all of them: every tile
