@@ -59,6 +59,8 @@ __global__ __launch_bounds__(256) void zero_kernel(double *p, int64_t n)
         p[i] = 0.0;
 }
 
+typedef double d2v __attribute__((ext_vector_type(2)));
+
 // out[i0,i1,i2,i3] = sum_t coef[t] * src_t[i0*s_t0 + i1*s_t1 + i2*s_t2 + i3*s_t3]
 // (terms added in list order).
 struct SumArgs {
@@ -68,6 +70,10 @@ struct SumArgs {
     const int64_t *strides;  // device, [nterms][4]
     const double *src[kMaxSumTerms];
     double coef[kMaxSumTerms];
+    // sum_plane_kernel<true>: terms whose element pairs (i3, i3 + 1) are
+    // adjacent and 16-byte aligned (unit i3 stride, even other strides,
+    // aligned buffer), one bit per term
+    uint64_t pairs[(kMaxSumTerms + 63) / 64];
 };
 
 __global__ __launch_bounds__(256) void sum_kernel(SumArgs a)
@@ -111,6 +117,12 @@ constexpr int kPlaneCap = 3072;  // doubles per staged plane: d3 rows of d2 + 1
 constexpr int kPlanePer = 12;    // result elements per thread: d2 * d3 <= 3072
 constexpr int kChunk = 4;        // terms per barrier
 
+// PAIRS (d2 and d3 even): each thread owns element pairs (i3, i3 + 1) of one
+// i2 -- six pairs for the 3072-element plane limit -- read with one 16-byte
+// load from the terms marked in `pairs` and stored with one 16-byte store;
+// every element still adds its terms in list order, with the same
+// operations as the element form.
+template <bool PAIRS>
 __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d3, FastDiv div_d2,
                                                         uint64_t nplanes, uint32_t bufstride)
 {
@@ -124,21 +136,28 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
         coef[threadIdx.x] = a.coef[threadIdx.x];
     }
     __syncthreads();
+    constexpr int W = PAIRS ? 2 : 1;          // elements per slot
+    constexpr int NS = kPlanePer / W;         // slots per thread
     const uint32_t d2 = (uint32_t)a.d2, d3 = (uint32_t)a.d3, P = d2 * d3, pitch = d2 + 1;
-    uint32_t i2[kPlanePer], i3[kPlanePer];
+    uint32_t i2[NS], i3[NS];
+    bool in[NS];
 #pragma unroll
-    for (int k = 0; k < kPlanePer; k++) {
-        const uint32_t e = min<uint32_t>(threadIdx.x + 256u * k, P - 1);
+    for (int k = 0; k < NS; k++) {
+        const uint32_t e0 = W * (threadIdx.x + 256u * k);
+        in[k] = e0 < P;
+        const uint32_t e = min<uint32_t>(e0, P - W);
         i2[k] = div_d3.div(e);
         i3[k] = e - i2[k] * d3;
     }
     for (uint64_t plane = blockIdx.x; plane < nplanes; plane += gridDim.x) {
-        const uint64_t i0 = plane / (uint64_t)a.d1, i1 = plane - i0 * (uint64_t)a.d1;
-        double acc[kPlanePer];
+        const uint64_t p0 = plane / (uint64_t)a.d1, p1 = plane - p0 * (uint64_t)a.d1;
+        double acc[NS][W];
 #pragma unroll
-        for (int k = 0; k < kPlanePer; k++) acc[k] = 0.0;
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+            for (int w = 0; w < W; w++) acc[k][w] = 0.0;
         for (int t0 = 0; t0 < a.nterms; t0 += kChunk) {
-            double xr[kChunk][kPlanePer];
+            double xr[kChunk][NS][W];
             bool staged[kChunk];
             __syncthreads();  // the previous chunk's staged planes have been read
 #pragma unroll
@@ -147,7 +166,7 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
                 staged[j] = false;
                 if (t >= a.nterms) continue;
                 const int64_t *q = st + 4 * t;
-                const double *x = src[t] + (int64_t)i0 * q[0] + (int64_t)i1 * q[1];
+                const double *x = src[t] + (int64_t)p0 * q[0] + (int64_t)p1 * q[1];
                 if (q[2] == 1 && q[3] != 1) {
                     staged[j] = true;
                     double *b = planes + j * bufstride;
@@ -155,12 +174,20 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
                         const uint32_t r3 = div_d2.div(idx), r2 = idx - r3 * d2;
                         b[r3 * pitch + r2] = x[r2 + (int64_t)r3 * q[3]];
                     }
+                } else if (PAIRS && ((a.pairs[t >> 6] >> (t & 63)) & 1)) {
+#pragma unroll
+                    for (int k = 0; k < NS; k++) {
+                        d2v v = in[k] ? *(const d2v *)(x + (int64_t)i2[k] * q[2] + i3[k]) : d2v{0.0, 0.0};
+                        xr[j][k][0] = v.x;
+                        xr[j][k][W - 1] = v.y;
+                    }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < kPlanePer; k++)
-                        xr[j][k] = threadIdx.x + 256u * k < P
-                                       ? x[(int64_t)i2[k] * q[2] + (int64_t)i3[k] * q[3]]
-                                       : 0.0;
+                    for (int k = 0; k < NS; k++)
+#pragma unroll
+                        for (int w = 0; w < W; w++)
+                            xr[j][k][w] = in[k] ? x[(int64_t)i2[k] * q[2] + (int64_t)(i3[k] + w) * q[3]]
+                                                : 0.0;
                 }
             }
             __syncthreads();
@@ -171,14 +198,20 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
                 const double c = coef[t];
                 const double *b = planes + j * bufstride;
 #pragma unroll
-                for (int k = 0; k < kPlanePer; k++)
-                    acc[k] += c * (staged[j] ? b[i3[k] * pitch + i2[k]] : xr[j][k]);
+                for (int k = 0; k < NS; k++)
+#pragma unroll
+                    for (int w = 0; w < W; w++)
+                        acc[k][w] += c * (staged[j] ? b[(i3[k] + w) * pitch + i2[k]] : xr[j][k][w]);
             }
         }
         double *o = a.out + plane * P;
 #pragma unroll
-        for (int k = 0; k < kPlanePer; k++)
-            if (threadIdx.x + 256u * k < P) o[threadIdx.x + 256u * k] = acc[k];
+        for (int k = 0; k < NS; k++) {
+            if (!in[k]) continue;
+            const uint32_t e = W * (threadIdx.x + 256u * k);
+            if constexpr (PAIRS) *(d2v *)(o + e) = d2v{acc[k][0], acc[k][1]};
+            else o[e] = acc[k][0];
+        }
     }
 }
 
@@ -318,6 +351,7 @@ struct PStmt {
     std::vector<double> coefs;
     int64_t od[4] = {1, 1, 1, 1};
     int64_t *dstrides = nullptr;  // device [nterms][4]
+    std::vector<int64_t> hstrides;  // host copy
     bool plane = false;           // sum_plane_kernel: (od[2] + 1) * od[3] <= kPlaneCap
     std::vector<int> deps;
     std::vector<int> succ;
@@ -624,18 +658,37 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
             static std::once_flag once;
             static hipError_t attr = hipSuccess;
             std::call_once(once, [] {
-                attr = hipFuncSetAttribute(reinterpret_cast<const void *>(sum_plane_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(kChunk * kPlaneCap * sizeof(double)));
+                for (const void *k : {reinterpret_cast<const void *>(sum_plane_kernel<false>),
+                                      reinterpret_cast<const void *>(sum_plane_kernel<true>)})
+                    if (attr == hipSuccess)
+                        attr = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)(kChunk * kPlaneCap * sizeof(double)));
             });
             DTCF_HIP(attr);
             const uint64_t nplanes = (uint64_t)(a.n / (s.od[2] * s.od[3]));
             const unsigned grid = (unsigned)std::min<uint64_t>(nplanes, 1u << 20);
             const uint32_t bufstride = (uint32_t)(s.od[3] * (s.od[2] + 1));
-            hipLaunchKernelGGL(sum_plane_kernel, dim3(grid), dim3(256),
-                               (size_t)kChunk * bufstride * sizeof(double), stream, a,
-                               FastDiv((uint32_t)s.od[3]), FastDiv((uint32_t)s.od[2]), nplanes,
-                               bufstride);
+            // element pairs when the plane's rows are whole pairs; per term, a
+            // 16-byte load where its pairs are adjacent and aligned
+            const bool pairs = s.od[2] % 2 == 0 && s.od[3] % 2 == 0 &&
+                               (reinterpret_cast<uintptr_t>(c) & 15) == 0;
+            if (pairs)
+                for (size_t t = 0; t < s.terms.size(); t++) {
+                    const int64_t *q = s.hstrides.data() + 4 * t;
+                    if (q[3] == 1 && q[0] % 2 == 0 && q[1] % 2 == 0 && q[2] % 2 == 0 &&
+                        (reinterpret_cast<uintptr_t>(a.src[t]) & 15) == 0)
+                        a.pairs[t >> 6] |= uint64_t(1) << (t & 63);
+                }
+            if (pairs)
+                hipLaunchKernelGGL(sum_plane_kernel<true>, dim3(grid), dim3(256),
+                                   (size_t)kChunk * bufstride * sizeof(double), stream, a,
+                                   FastDiv((uint32_t)s.od[3]), FastDiv((uint32_t)s.od[2]), nplanes,
+                                   bufstride);
+            else
+                hipLaunchKernelGGL(sum_plane_kernel<false>, dim3(grid), dim3(256),
+                                   (size_t)kChunk * bufstride * sizeof(double), stream, a,
+                                   FastDiv((uint32_t)s.od[3]), FastDiv((uint32_t)s.od[2]), nplanes,
+                                   bufstride);
         } else {
             hipLaunchKernelGGL(sum_kernel, dim3(grid_for(a.n)), dim3(256), 0, stream, a);
         }
@@ -1141,6 +1194,7 @@ extern "C" int dtcf_program_sum(dtcf_program p, const char *c, const char *c_idx
     if (st) return st;
     DTCF_HIP(hipMalloc(&s.dstrides, strides.size() * sizeof(int64_t)));
     DTCF_HIP(hipMemcpy(s.dstrides, strides.data(), strides.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    s.hstrides = strides;
     s.flops = (double)C.elems * nterms;
     s.plane = (s.od[2] + 1) * s.od[3] <= kPlaneCap && s.od[2] * s.od[3] <= 256 * kPlanePer;
     p->st.push_back(std::move(s));
