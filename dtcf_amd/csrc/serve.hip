@@ -114,7 +114,8 @@ __global__ __launch_bounds__(256) void sum_kernel(SumArgs a)
 // (lanes along i3: whole lines when its unit stride lies there).  (i2, i3)
 // come from 32-bit divisions by invariants, (i0, i1) once per plane.
 constexpr int kPlaneCap = 3072;  // doubles per staged plane: d3 rows of d2 + 1
-constexpr int kPlanePer = 12;    // result elements per thread: d2 * d3 <= 3072
+constexpr int kPlaneThreads = 512;
+constexpr int kPlanePer = 6;     // result elements per thread: d2 * d3 <= 3072
 constexpr int kChunk = 4;        // terms per barrier
 
 // PAIRS (d2 and d3 even): each thread owns element pairs (i3, i3 + 1) of one
@@ -123,14 +124,16 @@ constexpr int kChunk = 4;        // terms per barrier
 // every element still adds its terms in list order, with the same
 // operations as the element form.
 template <bool PAIRS>
-__global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d3, FastDiv div_d2,
-                                                        uint64_t nplanes, uint32_t bufstride)
+__global__ __launch_bounds__(kPlaneThreads, 4) void sum_plane_kernel(SumArgs a, FastDiv div_d3,
+                                                                  FastDiv div_d2, uint64_t nplanes,
+                                                                  uint32_t bufstride)
 {
+    constexpr uint32_t T = kPlaneThreads;
     extern __shared__ double planes[];  // kChunk staged planes of bufstride doubles
     __shared__ int64_t st[kMaxSumTerms * 4];
     __shared__ const double *src[kMaxSumTerms];
     __shared__ double coef[kMaxSumTerms];
-    for (int i = threadIdx.x; i < a.nterms * 4; i += 256) st[i] = a.strides[i];
+    for (int i = threadIdx.x; i < a.nterms * 4; i += T) st[i] = a.strides[i];
     if (threadIdx.x < a.nterms) {
         src[threadIdx.x] = a.src[threadIdx.x];
         coef[threadIdx.x] = a.coef[threadIdx.x];
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
     bool in[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) {
-        const uint32_t e0 = W * (threadIdx.x + 256u * k);
+        const uint32_t e0 = W * (threadIdx.x + T * k);
         in[k] = e0 < P;
         const uint32_t e = min<uint32_t>(e0, P - W);
         i2[k] = div_d3.div(e);
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
                 if (q[2] == 1 && q[3] != 1) {
                     staged[j] = true;
                     double *b = planes + j * bufstride;
-                    for (uint32_t idx = threadIdx.x; idx < P; idx += 256) {
+                    for (uint32_t idx = threadIdx.x; idx < P; idx += T) {
                         const uint32_t r3 = div_d2.div(idx), r2 = idx - r3 * d2;
                         b[r3 * pitch + r2] = x[r2 + (int64_t)r3 * q[3]];
                     }
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(256) void sum_plane_kernel(SumArgs a, FastDiv div_d
 #pragma unroll
         for (int k = 0; k < NS; k++) {
             if (!in[k]) continue;
-            const uint32_t e = W * (threadIdx.x + 256u * k);
+            const uint32_t e = W * (threadIdx.x + T * k);
             if constexpr (PAIRS) *(d2v *)(o + e) = d2v{acc[k][0], acc[k][1]};
             else o[e] = acc[k][0];
         }
@@ -680,12 +683,12 @@ int launch_stmt(dtcf_server srv, Req *r, int i, hipStream_t stream)
                         a.pairs[t >> 6] |= uint64_t(1) << (t & 63);
                 }
             if (pairs)
-                hipLaunchKernelGGL(sum_plane_kernel<true>, dim3(grid), dim3(256),
+                hipLaunchKernelGGL(sum_plane_kernel<true>, dim3(grid), dim3(kPlaneThreads),
                                    (size_t)kChunk * bufstride * sizeof(double), stream, a,
                                    FastDiv((uint32_t)s.od[3]), FastDiv((uint32_t)s.od[2]), nplanes,
                                    bufstride);
             else
-                hipLaunchKernelGGL(sum_plane_kernel<false>, dim3(grid), dim3(256),
+                hipLaunchKernelGGL(sum_plane_kernel<false>, dim3(grid), dim3(kPlaneThreads),
                                    (size_t)kChunk * bufstride * sizeof(double), stream, a,
                                    FastDiv((uint32_t)s.od[3]), FastDiv((uint32_t)s.od[2]), nplanes,
                                    bufstride);
@@ -1196,7 +1199,7 @@ extern "C" int dtcf_program_sum(dtcf_program p, const char *c, const char *c_idx
     DTCF_HIP(hipMemcpy(s.dstrides, strides.data(), strides.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     s.hstrides = strides;
     s.flops = (double)C.elems * nterms;
-    s.plane = (s.od[2] + 1) * s.od[3] <= kPlaneCap && s.od[2] * s.od[3] <= 256 * kPlanePer;
+    s.plane = (s.od[2] + 1) * s.od[3] <= kPlaneCap && s.od[2] * s.od[3] <= kPlaneThreads * kPlanePer;
     p->st.push_back(std::move(s));
     p->broken = false;
     return DTCF_OK;
