@@ -8,6 +8,23 @@
 
 namespace dtcf {
 
+// Sum of w[0], w[stride], ..., w[(n - 1) * stride] added in that order (the
+// K slices of a split product), with the loads issued eight at a time.
+__device__ inline double sum_in_order(const double *w, uint32_t n, int64_t stride)
+{
+    double sum = 0.0;
+    uint32_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = w[(int64_t)(i + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) sum += v[u];
+    }
+    for (; i < n; i++) sum += w[(int64_t)i * stride];
+    return sum;
+}
+
 // Thread-local last-error text, returned by dtcf_last_error().
 void set_error(const std::string &msg);
 const char *get_error();

@@ -507,9 +507,7 @@ __global__ __launch_bounds__(256) void group_split_reduce_kernel(GemmGroupArgs g
     for (int e = blockIdx.x * 256 + threadIdx.x; e < BM * BN; e += gridDim.x * 256) {
         const int64_t row = (int64_t)tm * BM + e / BN, col = (int64_t)tn * BN + e % BN;
         if (row >= P.m || col >= P.n) continue;
-        double sum = 0.0;
-        const double *w = ga.ws + (int64_t)ti * ga.sk * BM * BN + e;
-        for (uint32_t sl = 0; sl < ga.sk; sl++) sum += w[(int64_t)sl * BM * BN];
+        const double sum = sum_in_order(ga.ws + (int64_t)ti * ga.sk * BM * BN + e, ga.sk, BM * BN);
         const int64_t addr = row * P.ldc + col;
         if (ga.overwrite) P.C[addr] = ga.alpha * sum;
         else P.C[addr] += ga.alpha * sum;
@@ -527,9 +525,7 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(GemmArgs g)
     for (int e = blockIdx.x * 256 + threadIdx.x; e < BM * BN; e += gridDim.x * 256) {
         const int64_t row = (int64_t)tm * BM + e / BN, col = (int64_t)tn * BN + e % BN;
         if (row >= g.m || col >= g.n) continue;
-        double sum = 0.0;
-        const double *w = g.ws + (int64_t)t * g.sk * BM * BN + e;
-        for (uint32_t sl = 0; sl < g.sk; sl++) sum += w[(int64_t)sl * BM * BN];
+        const double sum = sum_in_order(g.ws + (int64_t)t * g.sk * BM * BN + e, g.sk, BM * BN);
         int64_t addr;
         if constexpr (CVIEW) addr = op_addr<kVK>(g.c, row, col);
         else addr = row * g.c.ld + col;
@@ -712,7 +708,7 @@ int launch_one(GemmArgs g, hipStream_t s)
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, g);
     DTCF_HIP(hipGetLastError());
     if (g.t_full < T) {
-        hipLaunchKernelGGL((split_reduce_kernel<BM, BN, CV>), dim3((BM * BN + 1023) / 1024, T - g.t_full),
+        hipLaunchKernelGGL((split_reduce_kernel<BM, BN, CV>), dim3((BM * BN + 255) / 256, T - g.t_full),
                            dim3(256), 0, s, g);
         DTCF_HIP(hipGetLastError());
     }
@@ -813,7 +809,7 @@ int launch_group(GemmGroupArgs ga, hipStream_t s)
     else hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, s, ga);
     DTCF_HIP(hipGetLastError());
     if (nsplit) {
-        hipLaunchKernelGGL((group_split_reduce_kernel<BM, BN>), dim3((BM * BN + 1023) / 1024, nsplit),
+        hipLaunchKernelGGL((group_split_reduce_kernel<BM, BN>), dim3((BM * BN + 255) / 256, nsplit),
                            dim3(256), 0, s, ga);
         DTCF_HIP(hipGetLastError());
     }

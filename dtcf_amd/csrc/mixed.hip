@@ -1213,9 +1213,7 @@ __global__ __launch_bounds__(256) void oz_split_reduce(OzArgs g)
     for (int e = blockIdx.x * 256 + threadIdx.x; e < OZ_BM * OZ_BN; e += gridDim.x * 256) {
         const int64_t row = m0 + e / OZ_BN, col = n0 + e % OZ_BN;
         if (row >= g.M || col >= g.N) continue;
-        const double *w = g.ws + (int64_t)q * g.sk * OZ_BM * OZ_BN + e;
-        double sum = 0.0;
-        for (uint32_t sl = 0; sl < g.sk; sl++) sum += w[(int64_t)sl * OZ_BM * OZ_BN];
+        const double sum = sum_in_order(g.ws + (int64_t)q * g.sk * OZ_BM * OZ_BN + e, g.sk, OZ_BM * OZ_BN);
         int64_t addr;
         if constexpr (CVIEW)
             addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + g.cv.cofs[col];
@@ -1245,7 +1243,7 @@ int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
     }
     DTCF_HIP(hipGetLastError());
     if (g.sk > 1) {
-        const dim3 rg(OZ_BM * OZ_BN / 1024, g.tiles_m * g.tiles_n);
+        const dim3 rg(OZ_BM * OZ_BN / 256, g.tiles_m * g.tiles_n);
         if (cview) hipLaunchKernelGGL(oz_split_reduce<true>, rg, dim3(256), 0, s, g);
         else hipLaunchKernelGGL(oz_split_reduce<false>, rg, dim3(256), 0, s, g);
         DTCF_HIP(hipGetLastError());
