@@ -829,10 +829,11 @@ struct OzArgs {
     double alpha;
     int64_t M, N, Kp, Mp, Np;    // Kp: bytes of K per slice row (a multiple of 64)
     int64_t kchunk_steps;        // K steps between INT32 flushes
-    // K split (grids with few tiles): workgroup b computes tile b / sk over
-    // K steps [(b % sk) * kslice, + kslice) and leaves alpha * its sum in
-    // ws[b] (OZ_BM x OZ_BN); oz_split_reduce adds the slices to C in order.
-    uint32_t sk;
+    // K split: workgroups [0, t_full) own whole tiles; workgroup t_full + s
+    // computes tile t_full + s / sk over K steps [(s % sk) * kslice, +
+    // kslice) and leaves alpha * its sum in ws[s] (OZ_BM x OZ_BN);
+    // oz_split_reduce adds the slices of tiles t_full.. to C in order.
+    uint32_t t_full, sk;
     int64_t kslice;
     double *ws;
     double *C;
@@ -979,13 +980,28 @@ __device__ __forceinline__ void oz_step(typename OzT<P>::Acc (&acc)[L - 1][OZ_FM
     }
 }
 
-// Tile of workgroup blockIdx.x: XCD-aware remap (consecutive tiles on one
-// XCD and its L2), then groups of 8 tile rows.
-__device__ __forceinline__ void oz_tile(const OzArgs &g, int64_t &m0, int64_t &n0)
+// Tile, K steps [k0, k0 + nk) and workspace tile (null: C itself) of
+// workgroup blockIdx.x.  Whole tiles take an XCD-aware remap (consecutive
+// tiles on one XCD and its L2); every tile index then goes through groups of
+// 8 tile rows.
+__device__ __forceinline__ void oz_tile(const OzArgs &g, int64_t &m0, int64_t &n0, int64_t &k0,
+                                        int64_t &nk, double *&wsp)
 {
-    const uint32_t T = g.tiles_m * g.tiles_n;
-    uint32_t q = blockIdx.x / g.sk;
-    if (g.sk == 1 && (T & 7u) == 0) q = (q & 7u) * (T >> 3) + (q >> 3);
+    const int64_t nkall = g.Kp / OZ_BK;
+    uint32_t q;
+    if (blockIdx.x < g.t_full) {
+        q = blockIdx.x;
+        if ((g.t_full & 7u) == 0) q = (q & 7u) * (g.t_full >> 3) + (q >> 3);
+        k0 = 0;
+        nk = nkall;
+        wsp = nullptr;
+    } else {
+        const uint32_t sidx = blockIdx.x - g.t_full;
+        q = g.t_full + sidx / g.sk;
+        k0 = (int64_t)(sidx % g.sk) * g.kslice;
+        nk = min(nkall - k0, g.kslice);
+        wsp = g.ws + (int64_t)sidx * OZ_BM * OZ_BN;
+    }
     const uint32_t width = 8 * g.tiles_n;
     const uint32_t first = (q / width) * 8;
     const uint32_t gsz = min(g.tiles_m - first, 8u);
@@ -1009,8 +1025,9 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wv / (OZ_BN / OZ_WN), wn = wv % (OZ_BN / OZ_WN);
 
-    int64_t m0, n0;
-    oz_tile(g, m0, n0);
+    int64_t m0, n0, k0, nk;
+    double *wsp;
+    oz_tile(g, m0, n0, k0, nk, wsp);
 
     const size_t pa = (size_t)g.Mp * g.Kp, pb = (size_t)g.Np * g.Kp;
     // Fragment f = wave * PER + c comes from a wave-uniform base (scalar
@@ -1048,7 +1065,6 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
     auto step = [&](const i4 *st) __attribute__((always_inline)) {
         oz_step<DTCF_PREC_I8, NS, L>(acc, st, st + FA * 64, wm, wn, lane);
     };
-    double *const wsp = g.ws ? g.ws + (int64_t)blockIdx.x * OZ_BM * OZ_BN : nullptr;
     bool first = true;
     int64_t since = 0;
     auto after = [&](bool more) __attribute__((always_inline)) {
@@ -1067,9 +1083,6 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_kernel(OzArgs g)
         }
     };
 
-    // this workgroup's K steps: [k0, k0 + nk)
-    const int64_t k0 = (int64_t)(blockIdx.x % g.sk) * g.kslice;
-    const int64_t nk = min(g.Kp / OZ_BK - k0, g.kslice);
     issue(stage0, k0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -1110,8 +1123,9 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wv / (OZ_BN / OZ_WN), wn = wv % (OZ_BN / OZ_WN);
-    int64_t m0, n0;
-    oz_tile(g, m0, n0);
+    int64_t m0, n0, k0, nk;
+    double *wsp;
+    oz_tile(g, m0, n0, k0, nk, wsp);
 
     const size_t pa = (size_t)g.Mp * g.Kp, pb = (size_t)g.Np * g.Kp;
     const int64_t nk64 = g.Kp / OZ_BK, lane_off = 16 * lane;
@@ -1140,10 +1154,6 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
     // s_waitcnt with vmcnt = PH and no expcnt / lgkmcnt wait (GFX9 encoding:
     // vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[15:14])
     constexpr int WAIT_PH = (PH & 15) | (7 << 4) | (15 << 8) | ((PH >> 4) << 14);
-    // this workgroup's K steps: [k0, k0 + nk) (k below counts from k0)
-    const int64_t k0 = (int64_t)(blockIdx.x % g.sk) * g.kslice;
-    const int64_t nk = min(g.Kp / OZ_BK - k0, g.kslice);
-    double *const wsp = g.ws ? g.ws + (int64_t)blockIdx.x * OZ_BM * OZ_BN : nullptr;
     bool first = true;
     int64_t since = 0;
     // Workgroup barrier without __syncthreads' release fence, which would
@@ -1204,7 +1214,7 @@ __global__ __launch_bounds__(64 * OZ_WAVES, 1) void ozaki_ring_kernel(OzArgs g)
 template <bool CVIEW>
 __global__ __launch_bounds__(256) void oz_split_reduce(OzArgs g)
 {
-    const uint32_t q = blockIdx.y;  // tile, in the order oz_tile gives a split launch
+    const uint32_t ti = blockIdx.y, q = g.t_full + ti;  // split tile, in oz_tile's order
     const uint32_t width = 8 * g.tiles_n;
     const uint32_t first = (q / width) * 8;
     const uint32_t gsz = min(g.tiles_m - first, 8u);
@@ -1213,7 +1223,7 @@ __global__ __launch_bounds__(256) void oz_split_reduce(OzArgs g)
     for (int e = blockIdx.x * 256 + threadIdx.x; e < OZ_BM * OZ_BN; e += gridDim.x * 256) {
         const int64_t row = m0 + e / OZ_BN, col = n0 + e % OZ_BN;
         if (row >= g.M || col >= g.N) continue;
-        const double sum = sum_in_order(g.ws + (int64_t)q * g.sk * OZ_BM * OZ_BN + e, g.sk, OZ_BM * OZ_BN);
+        const double sum = sum_in_order(g.ws + (int64_t)ti * g.sk * OZ_BM * OZ_BN + e, g.sk, OZ_BM * OZ_BN);
         int64_t addr;
         if constexpr (CVIEW)
             addr = g.cv.blk ? vaddr(g.cv, row, col) : g.cv.rofs[row] + g.cv.cofs[col];
@@ -1232,7 +1242,8 @@ int g_fused_i8 = 1;
 template <int P, int NS, int L>
 int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
 {
-    const unsigned grid = g.tiles_m * g.tiles_n * g.sk;
+    const uint32_t T = g.tiles_m * g.tiles_n;
+    const unsigned grid = g.t_full + (T - g.t_full) * g.sk;
     if constexpr (NS <= 4) {
         if (cview) hipLaunchKernelGGL((ozaki_ring_kernel<P, NS, L, true>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
         else hipLaunchKernelGGL((ozaki_ring_kernel<P, NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
@@ -1242,8 +1253,8 @@ int oz_launch(const OzArgs &g, bool cview, hipStream_t s)
         else hipLaunchKernelGGL((ozaki_kernel<NS, L, false>), dim3(grid), dim3(64 * OZ_WAVES), 0, s, g);
     }
     DTCF_HIP(hipGetLastError());
-    if (g.sk > 1) {
-        const dim3 rg(OZ_BM * OZ_BN / 256, g.tiles_m * g.tiles_n);
+    if (g.t_full < T) {
+        const dim3 rg(OZ_BM * OZ_BN / 256, T - g.t_full);
         if (cview) hipLaunchKernelGGL(oz_split_reduce<true>, rg, dim3(256), 0, s, g);
         else hipLaunchKernelGGL(oz_split_reduce<false>, rg, dim3(256), 0, s, g);
         DTCF_HIP(hipGetLastError());
@@ -1575,29 +1586,43 @@ int gemm_split_f64_impl(int prec, int nslice, int64_t m, int64_t n, int64_t k, d
         }
         const int64_t kmax = ((int64_t(1) << 31) - 1) / worst;
         o.kchunk_steps = prec == DTCF_PREC_I8 ? std::max<int64_t>(1, kmax / OZ_BK) : INT64_MAX;
-        // K split when the tiles fill under half the CUs (one workgroup per
-        // CU): sk slices of >= 16 K steps, sk * tiles <= CUs, so the slices
-        // (alpha * partial C tiles) fit the stream's GEMM split workspace.
+        // K split (one workgroup per CU): when the tiles fill under half the
+        // CUs, every tile is cut into sk slices of >= 16 K steps with sk *
+        // tiles <= CUs; when the last round of whole tiles fills at most half
+        // the CUs, its tiles alone are cut so that they fill one round (the
+        // 324 tiles of a 2304^2 result: 68 tail tiles x 3 slices after 256
+        // whole ones, instead of a second round 27% full).  nsplit * sk <=
+        // CUs, so the slices (alpha * partial C tiles) fit the stream's GEMM
+        // split workspace.
         const uint32_t tiles = o.tiles_m * o.tiles_n;
         const int64_t nkall = o.Kp / OZ_BK;
+        o.t_full = tiles;
         o.sk = 1;
         o.kslice = nkall;
         o.ws = nullptr;
         int cus = 0;
         int sst = gemm_device_cus(&cus);
         if (sst != DTCF_OK) return sst;
+        const uint32_t rem = tiles % (uint32_t)cus;
+        uint32_t nsplit = 0;
+        int64_t sk = 0;
         if (tiles < (uint32_t)cus / 2 && nkall >= 32) {
-            int64_t sk = std::min<int64_t>(cus / tiles, nkall / 16);
-            if (sk >= 2) {
-                const int64_t ks = (nkall + sk - 1) / sk;
-                sk = (nkall + ks - 1) / ks;
-                double *ws = nullptr;
-                sst = gemm_workspace(s, (size_t)tiles * sk * OZ_BM * OZ_BN * sizeof(double), &ws);
-                if (sst != DTCF_OK) return sst;
-                o.sk = (uint32_t)sk;
-                o.kslice = ks;
-                o.ws = ws;
-            }
+            nsplit = tiles;
+            sk = std::min<int64_t>(cus / tiles, nkall / 16);
+        } else if (tiles > (uint32_t)cus && rem > 0 && rem <= (uint32_t)cus / 2 && nkall >= 32) {
+            nsplit = rem;
+            sk = std::min<int64_t>(cus / rem, nkall / 16);
+        }
+        if (sk >= 2) {
+            const int64_t ks = (nkall + sk - 1) / sk;
+            sk = (nkall + ks - 1) / ks;
+            double *ws = nullptr;
+            sst = gemm_workspace(s, (size_t)nsplit * sk * OZ_BM * OZ_BN * sizeof(double), &ws);
+            if (sst != DTCF_OK) return sst;
+            o.t_full = tiles - nsplit;
+            o.sk = (uint32_t)sk;
+            o.kslice = ks;
+            o.ws = ws;
         }
         bool done = false;
         int st = oz_dispatch(prec, ns, lev, o, cv != nullptr, s, &done);

@@ -140,6 +140,36 @@ def test_fused_int8_matches_per_product(dev, ns, lev, m, n, k, cview):
     assert e <= {(5, 6): 5e-9, (4, 5): 5e-7, (4, 6): 5e-8, (5, 5): 1e-6, (3, 4): 3e-5, (4, 4): 3e-5}[(ns, lev)]
 
 
+@pytest.mark.parametrize("ns,lev", [(3, 4), (4, 6), (5, 6)])
+@pytest.mark.parametrize("m,n,k,cview", [(2176, 2048, 2048, False), (2304, 2304, 2048, True)])
+def test_fused_int8_tail_split_matches_per_product(dev, ns, lev, m, n, k, cview):
+    """Grids one partial round past the CU count (272 and 324 C tiles of
+    128 x 128 on 256 CUs): the tiles of the last round are cut into K slices
+    (after the whole ones) and summed in order by a second kernel; against
+    the per-product path, to FP64 rounding."""
+    rng = np.random.default_rng(m + n + k + ns)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((n, k))
+    C0 = rng.standard_normal((m, n))
+    Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    sa = mixed.Split(mixed.I8, Ad, mixed.dense_view(m, k), nslice=ns)
+    sb = mixed.Split(mixed.I8, Bd, mixed.dense_view(n, k), nslice=ns)
+    cv = ops.View(m, n, np.arange(m), np.arange(n) * m) if cview else None
+    out = []
+    try:
+        for fused in (True, False):
+            mixed.set_fused(fused)
+            Cd = torch.from_numpy((C0.T if cview else C0).copy().ravel()).to(dev)
+            mixed.gemm_split(sa, sb, Cd, alpha=-0.5, ldc=n, cview=cv, max_level=lev)
+            r = Cd.cpu().numpy()
+            out.append(r.reshape(n, m).T if cview else r.reshape(m, n))
+    finally:
+        mixed.set_fused(True)
+    d = relerr(out[0] - C0, out[1] - C0)
+    print(f"tail split ns={ns} L={lev} {m}x{n}x{k}: fused vs per-product {d:.1e}")
+    assert d <= 1e-13
+
+
 @pytest.mark.parametrize("prec", [mixed.BF16X3, mixed.F16X3])
 @pytest.mark.parametrize("m,n,k,cview", [(200, 136, 520, False), (64, 300, 96, True),
                                          (130, 260, 27000, False), (70, 130, 27000, True)])
