@@ -46,6 +46,11 @@ int pick_config(int64_t m, int64_t n, int64_t k)
     // few tiles split over K).  192 x 96 and 96 x 192 were never the
     // fastest (9216^3: 42.8 / 47.1 against 67.0 TF/s) and are used only
     // when forced.
+    // Thin results (one side at most 48) with short K stream their long
+    // operand past the thin one held in LDS (thin_f64_launch): 4.2 against
+    // 3.2 TB/s at 7077888 x 48 x 192, even at 442368 rows, slower than 64 x
+    // 64 tiles below about 262144 (110592 x 48 x 192: 75 against 72 us).
+    if (thin_shape(m, n, k, 262144)) return kCfgThin;
     if (k <= 1024) return kCfg64x64;
     auto padded = [&](int64_t t) { return (double)((m + t - 1) / t * t) * (double)((n + t - 1) / t * t); };
     return padded(128) <= 1.12 * padded(64) ? kCfg128x128 : kCfg64x64;
@@ -116,7 +121,7 @@ int dtcf_gemm_group_ex_f64(const dtcf_gemm_problem *probs, int nprob, int transa
                            double alpha, int overwrite, int cfg, void *stream)
 {
     DTCF_REQUIRE(nprob >= 0 && (probs || nprob == 0), "gemm group: null problem list");
-    DTCF_REQUIRE(cfg >= 0 && cfg < kCfgCount, "gemm group: unknown tile configuration");
+    DTCF_REQUIRE(cfg >= 0 && cfg < kCfgGroupCount, "gemm group: unknown tile configuration");
     DTCF_REQUIRE((transa == 0 || transa == 1) && (transb == 0 || transb == 1),
                  "gemm group: transa / transb must be 0 or 1");
     std::vector<int> order(nprob);

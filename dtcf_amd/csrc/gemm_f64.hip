@@ -581,6 +581,237 @@ __global__ __launch_bounds__(256) void gemv_kernel(GemvArgs g)
     }
 }
 
+// Thin products (ThinArgs): C (rows x nt, nt <= 48) = alpha * T S^T with T
+// the long operand (rows x k, k short) and S the thin one (nt x k).  Every
+// element of T is used once, so the product is bound by streaming T from
+// HBM (and by the FP64 MFMA rate, which at nt = 48 is about as fast); tiles
+// of 64 x 64 re-read S per tile and keep too few loads in flight (3 TB/s).
+// Here S lives in LDS for the whole launch, pre-arranged as the MFMA B
+// fragments of every 8-wide K chunk (two per chunk, t = 0, 1, one read of
+// 512 contiguous bytes each), and each wave streams units of 32 rows of T
+// through registers: a unit's K is read in super chunks of kThinSC chunks
+// (two 16-byte loads per lane per 8-wide chunk), the next super chunk (of
+// this unit or the wave's next) loading while this one runs its MFMAs.
+//   KP (k pairs adjacent, kDK / kVK): lane (i = l%16, q = l/16) loads
+//     T[r][k0 + 2q], T[r][k0 + 2q + 1] for r = row0 + i (row block 0) and
+//     r = row0 + 16 + i (block 1); component x pairs with fragment t = 0
+//     (k = k0 + 2q), y with t = 1.
+//   !KP (row pairs adjacent, kDMN / kVMN): lane loads T[row0 + 2i][k],
+//     T[row0 + 2i + 1][k] for k = k0 + q (t = 0) and k0 + 4 + q (t = 1);
+//     x feeds row block 0 (even rows), y block 1 (odd rows).
+// Both sum every C element over k in a fixed order, independent of timing.
+// No load's result is used in the item that issues it (the compiler's
+// in-order load counter would otherwise make every MFMA of that item wait
+// for the prefetch as well): the row offsets of a unit are loaded one item
+// ahead into one of two register sets, the long side's C offsets of the
+// unit being computed are loaded at the start of each item, the thin
+// side's C offsets sit in LDS, and every load is unconditional (past the
+// wave's last item and past K they repeat valid addresses).
+// Registers: two super chunks (96), 2 x NBB accumulators and the offsets;
+// NBB = 4 (nt up to 64) would spill at two waves per SIMD, so nt stops at 48.
+constexpr int kThinSC = 6;
+struct ThinArgs {
+    int64_t rows, nt, k;
+    double alpha;
+    const double *T;
+    OpArgs t;
+    int32_t tdense;  // T dense (ld) or a view without block table
+    const double *S;
+    OpArgs s;
+    int32_t smode;   // kDK, kDMN or a view mode
+    double *C;
+    OpArgs c;
+    int32_t cview, ctr;  // ctr: element (r, j) of the product is C(j, r); views have no block table
+    int32_t overwrite;
+};
+
+__device__ inline double thin_s(const ThinArgs &g, int64_t j, int64_t kk)
+{
+    if (g.smode == kDK) return g.S[j * g.s.ld + kk];
+    if (g.smode == kDMN) return g.S[kk * g.s.ld + j];
+    return g.S[op_addr<kVK>(g.s, j, kk)];
+}
+
+// TD: T dense; CV: C a view (compile-time, so no load sits in a branch whose
+// other side writes the same register, which would force a full wait).
+template <bool KP, int NBB, bool TD, bool CV>
+__global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
+{
+    extern __shared__ double smem[];
+    const int64_t K = g.k;
+    double *bf = smem;  // [K / 8][t][NBB][64 lanes]
+    int64_t *cp = reinterpret_cast<int64_t *>(smem + K * NBB * 16);  // column part of T per k
+    int64_t *cthin = cp + K;                                         // C offset per thin index
+    const int tid = threadIdx.x, lane = tid & 63, i16 = lane & 15, q = lane >> 4;
+
+    for (int64_t e = tid; e < K * NBB * 16; e += 256) {
+        const int l = (int)(e & 63);
+        const int64_t f = e >> 6;
+        const int cb = (int)(f % NBB), t = (int)((f / NBB) & 1);
+        const int64_t c = f / NBB / 2;
+        const int64_t j = 16 * cb + (l & 15), kk = KP ? 8 * c + 2 * (l >> 4) + t : 8 * c + 4 * t + (l >> 4);
+        bf[e] = j < g.nt ? thin_s(g, j, kk) : 0.0;
+    }
+    for (int64_t c = tid; c < K; c += 256) {
+        if constexpr (!TD) cp[c] = g.t.cofs[c];
+        else cp[c] = KP ? c : c * g.t.ld;
+    }
+    for (int j = tid; j < 16 * NBB; j += 256) {
+        const int64_t jj = min((int64_t)j, g.nt - 1);
+        if constexpr (CV) cthin[j] = g.ctr ? g.c.rofs[jj] : g.c.cofs[jj];
+        else cthin[j] = g.ctr ? jj * g.c.ld : jj;
+    }
+    __syncthreads();
+
+    const int64_t U = (g.rows + 31) / 32, NW = (int64_t)gridDim.x * 4;
+    int64_t cu = (int64_t)blockIdx.x * 4 + (tid >> 6);
+    if (cu >= U) return;
+    const int nsc = (int)((K + 8 * kThinSC - 1) / (8 * kThinSC));
+
+    // Row parts of T for unit u in this lane (rows past the end load the
+    // last row or row pair: in range, never stored).
+    auto row_part = [&](int64_t r) -> int64_t {
+        if constexpr (!TD) return g.t.rofs[r];
+        else return KP ? r * g.t.ld : r;
+    };
+    auto rows_of = [&](int64_t u, int64_t (&p)[2]) __attribute__((always_inline)) {
+        if constexpr (KP) {
+            p[0] = row_part(min(u * 32 + i16, g.rows - 1));
+            p[1] = row_part(min(u * 32 + 16 + i16, g.rows - 1));
+        } else {
+            p[0] = row_part(min(u * 32 + 2 * i16, g.rows - 2));
+            p[1] = 0;
+        }
+    };
+    // Row of C element r of row block b of unit u in this lane (MFMA
+    // result: lane l, element r holds local row q + 4r, column i16).
+    auto c_row = [&](int64_t u, int b, int r) -> int64_t {
+        return KP ? u * 32 + 16 * b + q + 4 * r : u * 32 + 2 * (q + 4 * r) + b;
+    };
+    auto c_offsets = [&](int64_t u, int64_t (&o)[2][4]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = min(c_row(u, b, r), g.rows - 1);
+                if constexpr (CV) o[b][r] = (g.ctr ? g.c.cofs : g.c.rofs)[row];
+                else o[b][r] = g.ctr ? row : row * g.c.ld;
+            }
+    };
+    auto load = [&](d2 (&a)[2 * kThinSC], const int64_t (&p)[2], int sc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < kThinSC; c++) {
+            const int64_t k0 = min((int64_t)(8 * kThinSC * sc + 8 * c), K - 8);
+            if constexpr (KP) {
+                const int64_t co = cp[k0 + 2 * q];
+                a[2 * c] = *reinterpret_cast<const d2 *>(g.T + p[0] + co);
+                a[2 * c + 1] = *reinterpret_cast<const d2 *>(g.T + p[1] + co);
+            } else {
+                a[2 * c] = *reinterpret_cast<const d2 *>(g.T + p[0] + cp[k0 + q]);
+                a[2 * c + 1] = *reinterpret_cast<const d2 *>(g.T + p[0] + cp[k0 + 4 + q]);
+            }
+        }
+    };
+    d4 acc[2][NBB];
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int j = 0; j < NBB; j++) acc[b][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    auto compute = [&](const d2 (&a)[2 * kThinSC], int sc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < kThinSC; c++) {
+            const int64_t kc = kThinSC * sc + c;
+            if (8 * kc >= K) break;
+#pragma unroll
+            for (int j = 0; j < NBB; j++) {
+                const double b0 = bf[((kc * 2) * NBB + j) * 64 + lane];
+                const double b1 = bf[((kc * 2 + 1) * NBB + j) * 64 + lane];
+                if constexpr (KP) {
+                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].x, b0, acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].x, b0, acc[1][j], 0, 0, 0);
+                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].y, b1, acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].y, b1, acc[1][j], 0, 0, 0);
+                } else {
+                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].x, b0, acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].y, b0, acc[1][j], 0, 0, 0);
+                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].x, b1, acc[0][j], 0, 0, 0);
+                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].y, b1, acc[1][j], 0, 0, 0);
+                }
+            }
+        }
+    };
+    // Accumulating into C reads a row block's old values with one batch of
+    // loads (clamped addresses, all in range) and one wait.
+    auto store = [&](int64_t u, const int64_t (&o)[2][4]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            double old[NBB][4];
+            if (!g.overwrite) {
+#pragma unroll
+                for (int j = 0; j < NBB; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) old[j][r] = g.C[o[b][r] + cthin[16 * j + i16]];
+            }
+#pragma unroll
+            for (int j = 0; j < NBB; j++) {
+                const int64_t col = 16 * j + i16;
+                const int64_t co = cthin[16 * j + i16];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (c_row(u, b, r) >= g.rows || col >= g.nt) continue;
+                    const double v = g.alpha * acc[b][j][r];
+                    g.C[o[b][r] + co] = g.overwrite ? v : old[j][r] + v;
+                }
+                acc[b][j] = (d4){0.0, 0.0, 0.0, 0.0};
+            }
+        }
+    };
+
+    // The wave's items (unit, super chunk) in order; item n + 1 loads while
+    // item n computes.  p: row parts of the unit being loaded; qa / qb: those
+    // of the unit after it, loaded by alternate items and used one item
+    // later.  Unit indices past the end are clamped, so every item issues
+    // the same loads.
+    int64_t p[2], qa[2], qb[2], co[2][4];
+    rows_of(cu, p);
+    rows_of(min(cu + NW, U - 1), qb);
+    d2 ra[2 * kThinSC], rb[2 * kThinSC];
+    load(ra, p, 0);
+    int cs = 0;
+    auto next = [&](d2 (&a)[2 * kThinSC], int64_t (&qload)[2], const int64_t (&quse)[2], int64_t &lu,
+                    int &ls) __attribute__((always_inline)) {
+        c_offsets(cu, co);
+        lu = cu;
+        ls = cs + 1;
+        if (ls == nsc) {
+            ls = 0;
+            lu += NW;
+        }
+        const bool nu = ls == 0;
+        p[0] = nu ? quse[0] : p[0];
+        p[1] = nu ? quse[1] : p[1];
+        rows_of(min(lu + NW, U - 1), qload);
+        load(a, p, ls);
+    };
+    while (true) {
+        int64_t lu;
+        int ls;
+        next(rb, qa, qb, lu, ls);
+        compute(ra, cs);
+        if (cs == nsc - 1) store(cu, co);
+        if (lu >= U) break;
+        cu = lu;
+        cs = ls;
+
+        next(ra, qb, qa, lu, ls);
+        compute(rb, cs);
+        if (cs == nsc - 1) store(cu, co);
+        if (lu >= U) break;
+        cu = lu;
+        cs = ls;
+    }
+}
+
 // CU count per device, and the tail-split workspace per (device, stream):
 // GEMMs enqueued on different streams never share a workspace; GEMMs on one
 // stream are ordered by it.
@@ -864,6 +1095,86 @@ int gemv_f64_launch(const GemvArgs &g, hipStream_t s)
     return DTCF_OK;
 }
 
+bool thin_shape(int64_t m, int64_t n, int64_t k, int64_t min_rows)
+{
+    const int64_t w = std::min(m, n), rows = std::max(m, n);
+    if (w > kThinMaxWidth || rows < min_rows || k <= 0 || k % 8 != 0) return false;
+    const int64_t nbb = (w + 15) / 16;
+    return (k * (16 * nbb + 1) + 16 * nbb) * (int64_t)sizeof(double) <= kThinLdsBytes;
+}
+
+template <bool KP, int NBB, bool TD, bool CV>
+int thin_launch_t(const ThinArgs &t, hipStream_t s)
+{
+    auto kern = thin_kernel<KP, NBB, TD, CV>;
+    static std::once_flag once;
+    static hipError_t attr_err = hipSuccess;
+    std::call_once(once, [&] {
+        attr_err = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kThinLdsBytes);
+    });
+    DTCF_HIP(attr_err);
+    int cus = 0;
+    const int st = device_cus(&cus);
+    if (st != DTCF_OK) return st;
+    // Two resident workgroups per CU (LDS up to 80 KiB each), each wave
+    // looping over 32-row units.
+    const int64_t units = (t.rows + 31) / 32;
+    const unsigned grid = (unsigned)std::min<int64_t>((units + 3) / 4, (int64_t)cus * 2);
+    const size_t lds = ((size_t)t.k * (16 * NBB + 1) + 16 * NBB) * sizeof(double);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, t);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
+}
+
+template <bool KP, bool TD>
+int thin_dispatch(const ThinArgs &t, int nbb, hipStream_t s)
+{
+    if (t.cview) {
+        if (nbb == 1) return thin_launch_t<KP, 1, TD, true>(t, s);
+        if (nbb == 2) return thin_launch_t<KP, 2, TD, true>(t, s);
+        return thin_launch_t<KP, 3, TD, true>(t, s);
+    }
+    if (nbb == 1) return thin_launch_t<KP, 1, TD, false>(t, s);
+    if (nbb == 2) return thin_launch_t<KP, 2, TD, false>(t, s);
+    return thin_launch_t<KP, 3, TD, false>(t, s);
+}
+
+int thin_f64_launch(const GemmArgs &g, hipStream_t s)
+{
+    // The long operand: A when C is thin in n, else B (C written transposed).
+    const bool tall_a = g.n <= kThinMaxWidth && g.m >= g.n;
+    ThinArgs t{};
+    t.rows = tall_a ? g.m : g.n;
+    t.nt = tall_a ? g.n : g.m;
+    t.k = g.k;
+    if (!thin_shape(t.rows, t.nt, t.k, 1)) return kThinDecline;
+    const OpArgs &to = tall_a ? g.a : g.b;
+    const int tmode = tall_a ? g.amode : g.bmode;
+    const bool kp = tmode == kDK || tmode == kVK;
+    if (!to.vec2 || (tmode >= kVK && to.blk) || (!kp && t.rows % 2 != 0)) return kThinDecline;
+    if (g.cview && g.c.blk) return kThinDecline;
+    t.alpha = g.alpha;
+    t.T = tall_a ? g.A : g.B;
+    t.t = to;
+    t.tdense = tmode <= kDMN;
+    t.S = tall_a ? g.B : g.A;
+    t.s = tall_a ? g.b : g.a;
+    t.smode = tall_a ? g.bmode : g.amode;
+    t.C = g.C;
+    t.c = g.c;
+    t.cview = g.cview;
+    t.ctr = tall_a ? 0 : 1;
+    t.overwrite = g.overwrite;
+    const int nbb = (int)((t.nt + 15) / 16);
+    if (kp) {
+        if (t.tdense) return thin_dispatch<true, true>(t, nbb, s);
+        return thin_dispatch<true, false>(t, nbb, s);
+    }
+    if (t.tdense) return thin_dispatch<false, true>(t, nbb, s);
+    return thin_dispatch<false, false>(t, nbb, s);
+}
+
 int gemm_workspace(hipStream_t s, size_t need, double **ws) { return split_workspace(s, need, ws); }
 
 int gemm_device_cus(int *cus) { return device_cus(cus); }
@@ -876,7 +1187,7 @@ int gemm_reserve_workspace(hipStream_t s)
     int cus = 0;
     int st = device_cus(&cus);
     if (st != DTCF_OK) return st;
-    const size_t tile = std::max<size_t>({128 * 128, 192 * 96, 96 * 192, 64 * 64});
+    const size_t tile = std::max<size_t>({128 * 128, 192 * 96, 96 * 192, 64 * 64, 192 * 64});
     double *ws = nullptr;
     return split_workspace(s, (size_t)cus * 2 * 2 * tile * sizeof(double), &ws);
 }
@@ -890,6 +1201,13 @@ int gemm_f64_launch(GemmArgs g, int cfg, hipStream_t s)
     case kCfg192x96: return launch_cfg<192, 96, 96, 48>(g, s);
     case kCfg96x192: return launch_cfg<96, 192, 48, 96>(g, s);
     case kCfg64x64: return launch_cfg<64, 64, 32, 32>(g, s);
+    case kCfg192x64: return launch_cfg<192, 64, 48, 64>(g, s);
+    case kCfg64x192: return launch_cfg<64, 192, 64, 48>(g, s);
+    case kCfgThin: {
+        const int st = thin_f64_launch(g, s);
+        if (st != kThinDecline) return st;
+        return launch_cfg<64, 64, 32, 32>(g, s);
+    }
     default: break;
     }
     set_error("gemm: unknown tile configuration");

@@ -15,7 +15,23 @@ namespace dtcf {
 enum { kDK = 0, kDMN = 1, kVK = 2, kVMN = 3 };
 
 // Tile configurations (BM x BN), selectable per launch by the autotuner.
-enum { kCfg128x128 = 0, kCfg192x96 = 1, kCfg96x192 = 2, kCfg64x64 = 3, kCfgCount = 4 };
+// 192x64 and 64x192 serve thin results (one side at most 64) with long K:
+// a 64-wide tile of the thin side, three times as deep on the other, so
+// the thin operand is re-read by a third as many tiles as with 64 x 64.
+// kCfgThin is the streaming kernel for thin results (one side at most 48)
+// with short K (thin_f64_launch); shapes it does not take run 64 x 64
+// tiles.  The grouped launch takes the first four.
+enum {
+    kCfg128x128 = 0,
+    kCfg192x96 = 1,
+    kCfg96x192 = 2,
+    kCfg64x64 = 3,
+    kCfg192x64 = 4,
+    kCfg64x192 = 5,
+    kCfgThin = 6,
+    kCfgCount = 7,
+    kCfgGroupCount = 4
+};
 // The C tile configuration dtcf_gemm_* uses for an m x n x k GEMM.
 int pick_config(int64_t m, int64_t n, int64_t k);
 
@@ -101,6 +117,18 @@ struct GemvArgs {
     int64_t ldy;
 };
 int gemv_f64_launch(const GemvArgs &g, hipStream_t s);
+
+// Thin products: one side of C at most kThinMaxWidth wide, the other long
+// (rows), K a multiple of 8 and short enough that the thin operand's K x
+// width fragments and the long operand's K column offsets fit
+// kThinLdsBytes of LDS.  thin_shape is the shape test; the launch also
+// needs the long operand's pairs adjacent and 16-byte aligned (vec2) and no
+// block table, and returns kThinDecline otherwise.
+constexpr int64_t kThinMaxWidth = 48;
+constexpr int64_t kThinLdsBytes = 80 * 1024;
+constexpr int kThinDecline = 1;
+bool thin_shape(int64_t m, int64_t n, int64_t k, int64_t min_rows);
+int thin_f64_launch(const GemmArgs &g, hipStream_t s);
 
 // Allocates the tail-split workspace of stream `s` at the largest size any
 // GEMM on it can need, so later launches on `s` never allocate.

@@ -303,7 +303,8 @@ class Program:
     PLAN_FLAGS = {1: "swap", 2: "mixed", 4: "pack_a", 8: "pack_b", 16: "a_colpairs",
                   32: "a_rowpairs", 64: "b_colpairs", 128: "b_rowpairs", 256: "c_dense",
                   512: "res_a", 1024: "res_b", 2048: "sum_plane"}
-    TILES = {-1: "mixed", 0: "128x128", 1: "192x96", 2: "96x192", 3: "64x64"}
+    TILES = {-1: "mixed", 0: "128x128", 1: "192x96", 2: "96x192", 3: "64x64", 4: "192x64", 5: "64x192",
+             6: "thin"}
 
     def plan(self, i):
         """The finalized plan of statement i (dtcf_program_statement_plan):

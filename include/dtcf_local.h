@@ -141,7 +141,9 @@ int dtcf_view_scatter_f64(const double *src, int64_t ld, dtcf_view v, double *ds
 int dtcf_gemm_reserve_workspace(void *stream);
 
 /* Tile-shape selection for the GEMM kernels: 0 = 128x128, 1 = 192x96,
- * 2 = 96x192, 3 = 64x64, -1 = automatic (autotuned table). */
+ * 2 = 96x192, 3 = 64x64, 4 = 192x64, 5 = 64x192, 6 = the thin streaming
+ * kernel (one side of C at most 48 wide, K a multiple of 8 and at most
+ * about 200; other shapes take 64x64), -1 = automatic (autotuned table). */
 int dtcf_gemm_set_config(int cfg);
 
 /* Host-pointer variant of dtcf_gemm_f64 for the drop-in local_contraction:
@@ -567,7 +569,8 @@ int dtcf_program_info(dtcf_program p, int64_t *block_bytes, int64_t *nstatements
  * m, n, k (operands possibly swapped), DTCF_PLAN_* flags (operand access:
  * column or row pairs contiguous, packed into a dense copy first, resident
  * slices; mixed precision; C written densely) and the FP64 C tile
- * configuration (0 128x128, 1 192x96, 2 96x192, 3 64x64; -1 when mixed);
+ * configuration (0 128x128, 1 192x96, 2 96x192, 3 64x64, 4 192x64,
+ * 5 64x192, 6 thin streaming kernel; -1 when mixed);
  * for a sum the number of terms and whether it stages planes in LDS. */
 enum {
     DTCF_PLAN_KIND = 0, DTCF_PLAN_M = 1, DTCF_PLAN_N = 2, DTCF_PLAN_K = 3, DTCF_PLAN_FLAGS = 4,

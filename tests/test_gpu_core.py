@@ -96,7 +96,7 @@ def test_mfma_layout_single_block(dev):
     assert relerr(got, ref) <= GEMM_TOL
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("at,bt", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_gemm_f64_shapes(dev, cfg, at, bt):
     ops.set_gemm_config(cfg)
@@ -373,6 +373,74 @@ def test_gemm_view_random_shapes(dev):
         got = unscr(Cd.cpu().numpy())
         ref = C0 + alpha * (Ah @ Bh.T)
         assert relerr(got, ref) <= GEMM_TOL, (case, m, n, k)
+
+
+# (m, n, k, transa, transb): thin in n (the long operand A) or in m (B,
+# C written transposed); long operand with k pairs adjacent (A untransposed,
+# B transposed) or row pairs adjacent, odd long sides (row pairs: the tile
+# path instead), widths 1-48, K from 8 to 200 (the largest at width 48 that
+# fits LDS), and shapes the thin kernel declines (width 49, K not a multiple
+# of 8, K too long).
+THIN_SHAPES = [(20000, 48, 192, 0, 0), (20000, 48, 192, 1, 1), (3001, 17, 8, 0, 1), (1000, 16, 48, 1, 0),
+               (1001, 1, 16, 1, 0), (48, 5000, 200, 0, 0), (40, 3001, 64, 0, 1), (17000, 32, 96, 1, 1),
+               (64, 40, 8, 0, 0), (5, 3, 24, 0, 1), (2, 7000, 48, 1, 1), (4000, 49, 64, 0, 0),
+               (3000, 40, 12, 0, 0), (3000, 40, 2048, 0, 1)]
+
+
+@pytest.mark.parametrize("cfg", [6, -1])
+def test_gemm_thin_dense(dev, cfg):
+    """Thin products through the dense GEMM, the streaming kernel forced
+    (cfg 6: every shape it takes, the rest on 64 x 64 tiles) or by the
+    automatic choice (long side >= 16384), C accumulated, against numpy."""
+    rng = np.random.default_rng(77)
+    ops.set_gemm_config(cfg)
+    try:
+        for (m, n, k, at, bt) in THIN_SHAPES:
+            say("thin dense", cfg, m, n, k, at, bt)
+            A = rng.standard_normal((k, m) if at else (m, k))
+            B = rng.standard_normal((n, k) if bt else (k, n))
+            C0 = rng.standard_normal((m, n))
+            Cd = torch.from_numpy(C0.copy()).to(dev)
+            ops.gemm(at, bt, m, n, k, 0.75, torch.from_numpy(A).to(dev), A.shape[1],
+                     torch.from_numpy(B).to(dev), B.shape[1], Cd, n)
+            ref = C0 + 0.75 * ((A.T if at else A) @ (B.T if bt else B))
+            assert relerr(Cd.cpu().numpy(), ref) <= GEMM_TOL, (m, n, k, at, bt)
+    finally:
+        ops.set_gemm_config(-1)
+
+
+def test_gemm_thin_views(dev):
+    """Thin products through the view GEMM with the streaming kernel forced:
+    operands in random layouts (row-major, column-major, scrambled rows, no
+    adjacent pairs), C dense with a pitch or a scrambled-row view, thin in n
+    or in m, against numpy."""
+    rng = np.random.default_rng(4242)
+    shapes = [(3000, 48, 192), (2998, 33, 48), (40, 2500, 200), (1999, 1, 8), (16, 4000, 64), (4001, 20, 8)]
+    ops.set_gemm_config(6)
+    try:
+        for case in range(24):
+            m, n, k = shapes[case % len(shapes)]
+            Ah, Ad, av = _random_layout(rng, m, k, dev)
+            Bh, Bd, bv = _random_layout(rng, n, k, dev)
+            C0 = rng.standard_normal((m, n))
+            if case % 2:
+                rp = rng.permutation(m)
+                Cd = torch.from_numpy(C0[np.argsort(rp)].ravel().copy()).to(dev)
+                cv = ops.View(m, n, (rp * n).astype(np.int64), np.arange(n, dtype=np.int64))
+                ldc, unscr = n, (lambda x, rp=rp, m=m, n=n: x.reshape(m, n)[rp])
+            else:
+                ldc = n + 1
+                Cbuf = np.zeros(m * ldc)
+                Cbuf.reshape(m, ldc)[:, :n] = C0
+                Cd, cv = torch.from_numpy(Cbuf).to(dev), None
+                unscr = lambda x, m=m, n=n, ldc=ldc: x.reshape(m, ldc)[:, :n]
+            say("thin view", case, m, n, k, cv is not None)
+            ops.gemm_view(m, n, k, -1.25, Ad, av, Bd, bv, Cd, cv, ldc)
+            torch.cuda.synchronize()
+            ref = C0 - 1.25 * (Ah @ Bh.T)
+            assert relerr(unscr(Cd.cpu().numpy()), ref) <= GEMM_TOL, (case, m, n, k)
+    finally:
+        ops.set_gemm_config(-1)
 
 
 @pytest.mark.parametrize("m,n,k", [(2200, 2100, 520), (2049, 2300, 1040)])

@@ -6,7 +6,7 @@ import pytest
 import torch
 
 import oracle
-from dtcf_amd import mixed, termlist
+from dtcf_amd import mixed, ops, termlist
 from dtcf_amd import serve as S
 from dtcf_amd._lib import lib
 
@@ -54,6 +54,31 @@ def test_every_statement_matches_oracle(dev, lists, which):
             assert err <= 1e-12, (name, err)
         print(f"  {len(prog.outputs)} arrays, worst relative error {worst:.2e}")
         prog.destroy()
+
+
+@pytest.mark.parametrize("which", ["t1", "t2"])
+def test_every_statement_matches_oracle_thin_kernel(dev, lists, which):
+    """Every statement with the thin streaming kernel forced for the FP64
+    contractions it takes (one side at most 48, K a multiple of 8: with
+    these extents most contractions over an o-v or v-v pair), C written
+    fresh (overwrite) through the statement's views, against the oracle."""
+    tl = lists[which]
+    sizes = {"Oa": 4, "Ob": 2, "Va": 8, "Vb": 6}
+    print(f"serve parity thin {which} sizes {sizes}", flush=True)
+    ops.set_gemm_config(6)
+    try:
+        with S.Server(1 << 30, nslots=4) as srv:
+            sysm = S.System(srv, 0, 0, [tl], sizes=sizes)
+            prog = S.Program(srv, tl, sysm, expose=True)
+            got = prog.run()
+            ref = oracle_run(tl, sysm)
+            for name in prog.outputs:
+                r = ref[name]
+                err = np.abs(got[name] - r).max() / max(np.abs(r).max(), 1e-300)
+                assert err <= 1e-12, (name, err)
+            prog.destroy()
+    finally:
+        ops.set_gemm_config(-1)
 
 
 @pytest.mark.parametrize("layout", [S.LAYOUT_REUSE, S.LAYOUT_SEPARATE])
@@ -372,7 +397,7 @@ def test_statement_plans(dev, lists):
                 assert pl["k"] == ek and sorted((pl["m"], pl["n"])) == sorted((ea, eb)), (s.lhs.name, pl)
                 mixed_rule = prec != 0 and pl["m"] * pl["n"] >= 2 * (pl["m"] + pl["n"])
                 assert ("mixed" in pl["flags"]) == mixed_rule, (s.lhs.name, pl)
-                assert pl["tile"] in (("mixed",) if mixed_rule else ("128x128", "64x64")), pl
+                assert pl["tile"] in (("mixed",) if mixed_rule else ("128x128", "64x64", "thin")), pl
             if prec:
                 assert any("mixed" in prog.plan(i)["flags"] for i in range(len(tl.statements)))
             prog.destroy()
