@@ -581,17 +581,20 @@ __global__ __launch_bounds__(256) void gemv_kernel(GemvArgs g)
     }
 }
 
-// Thin products (ThinArgs): C (rows x nt, nt <= 48) = alpha * T S^T with T
-// the long operand (rows x k, k short) and S the thin one (nt x k).  Every
-// element of T is used once, so the product is bound by streaming T from
-// HBM (and by the FP64 MFMA rate, which at nt = 48 is about as fast); tiles
-// of 64 x 64 re-read S per tile and keep too few loads in flight (3 TB/s).
-// Here S lives in LDS for the whole launch, pre-arranged as the MFMA B
+// Thin products (ThinArgs): C (rows x nt) = alpha * T S^T with T the long
+// operand (rows x k, k short) and S the thin one (nt x k; nt <= 48, or up
+// to 192 when k <= 48).  Every element of T is used once, so the product
+// is bound by streaming T from HBM, by the FP64 MFMA rate (about as fast at
+// nt = 48) or, at wide nt and short k, by writing C; tiles of 64 x 64
+// re-read S per tile and keep too few loads in flight (3 TB/s).
+// Here S lives in LDS for the whole launch, pre-arranged as the MFMA
 // fragments of every 8-wide K chunk (two per chunk, t = 0, 1, one read of
-// 512 contiguous bytes each), and each wave streams units of 32 rows of T
-// through registers: a unit's K is read in super chunks of kThinSC chunks
-// (two 16-byte loads per lane per 8-wide chunk), the next super chunk (of
-// this unit or the wave's next) loading while this one runs its MFMAs.
+// 512 contiguous bytes each) and 16-wide block of S rows, and each wave
+// streams units of 32 rows of T through registers: a unit's K is read in
+// super chunks of kThinSC chunks (two 16-byte loads per lane per 8-wide
+// chunk), the next super chunk (of this unit or the wave's next) loading
+// while this one runs its MFMAs.  With nt > 48 (k <= 48: one super chunk)
+// the unit's registers serve ng groups of 48 S rows in turn.
 //   KP (k pairs adjacent, kDK / kVK): lane (i = l%16, q = l/16) loads
 //     T[r][k0 + 2q], T[r][k0 + 2q + 1] for r = row0 + i (row block 0) and
 //     r = row0 + 16 + i (block 1); component x pairs with fragment t = 0
@@ -599,19 +602,25 @@ __global__ __launch_bounds__(256) void gemv_kernel(GemvArgs g)
 //   !KP (row pairs adjacent, kDMN / kVMN): lane loads T[row0 + 2i][k],
 //     T[row0 + 2i + 1][k] for k = k0 + q (t = 0) and k0 + 4 + q (t = 1);
 //     x feeds row block 0 (even rows), y block 1 (odd rows).
+// The register fragment is the MFMA's A operand (T rows down the result,
+// S rows across: lanes of a store run along S) or, TC, its B operand (the
+// result transposed: lanes run along T), chosen so a store's 16 lanes
+// write consecutive addresses of C: TC when C(r, j) is C's column r (the
+// product of a thin A, written transposed).
 // Both sum every C element over k in a fixed order, independent of timing.
 // No load's result is used in the item that issues it (the compiler's
 // in-order load counter would otherwise make every MFMA of that item wait
 // for the prefetch as well): the row offsets of a unit are loaded one item
-// ahead into one of two register sets, the long side's C offsets of the
-// unit being computed are loaded at the start of each item, the thin
-// side's C offsets sit in LDS, and every load is unconditional (past the
-// wave's last item and past K they repeat valid addresses).
+// ahead into one of two register sets, the T side's C offsets of the unit
+// being computed are loaded at the start of each item, the S side's sit in
+// LDS, and every load is unconditional (past the wave's last item and past
+// K they repeat valid addresses).
 // Registers: two super chunks (96), 2 x NBB accumulators and the offsets;
-// NBB = 4 (nt up to 64) would spill at two waves per SIMD, so nt stops at 48.
+// NBB = 4 would spill at two waves per SIMD, so a group is 48 wide.
 constexpr int kThinSC = 6;
 struct ThinArgs {
     int64_t rows, nt, k;
+    int32_t ng;  // groups of 16 * NBB rows of S
     double alpha;
     const double *T;
     OpArgs t;
@@ -632,23 +641,25 @@ __device__ inline double thin_s(const ThinArgs &g, int64_t j, int64_t kk)
     return g.S[op_addr<kVK>(g.s, j, kk)];
 }
 
-// TD: T dense; CV: C a view (compile-time, so no load sits in a branch whose
-// other side writes the same register, which would force a full wait).
-template <bool KP, int NBB, bool TD, bool CV>
+// TD: T dense; CV: C a view; TC: transposed C (ctr), the transposed MFMA
+// form.  Compile-time, so no load sits in a branch whose other side writes
+// the same register, which would force a full wait.
+template <bool KP, int NBB, bool TD, bool CV, bool TC>
 __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
 {
     extern __shared__ double smem[];
     const int64_t K = g.k;
-    double *bf = smem;  // [K / 8][t][NBB][64 lanes]
-    int64_t *cp = reinterpret_cast<int64_t *>(smem + K * NBB * 16);  // column part of T per k
-    int64_t *cthin = cp + K;                                         // C offset per thin index
+    const int NBT = g.ng * NBB;  // 16-row blocks of S
+    double *bf = smem;           // [K / 8][t][NBT][64 lanes]
+    int64_t *cp = reinterpret_cast<int64_t *>(smem + K * NBT * 16);  // column part of T per k
+    int64_t *cthin = cp + K;                                         // C offset per S row
     const int tid = threadIdx.x, lane = tid & 63, i16 = lane & 15, q = lane >> 4;
 
-    for (int64_t e = tid; e < K * NBB * 16; e += 256) {
+    for (int64_t e = tid; e < K * NBT * 16; e += 256) {
         const int l = (int)(e & 63);
         const int64_t f = e >> 6;
-        const int cb = (int)(f % NBB), t = (int)((f / NBB) & 1);
-        const int64_t c = f / NBB / 2;
+        const int cb = (int)(f % NBT), t = (int)((f / NBT) & 1);
+        const int64_t c = f / NBT / 2;
         const int64_t j = 16 * cb + (l & 15), kk = KP ? 8 * c + 2 * (l >> 4) + t : 8 * c + 4 * t + (l >> 4);
         bf[e] = j < g.nt ? thin_s(g, j, kk) : 0.0;
     }
@@ -656,10 +667,10 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
         if constexpr (!TD) cp[c] = g.t.cofs[c];
         else cp[c] = KP ? c : c * g.t.ld;
     }
-    for (int j = tid; j < 16 * NBB; j += 256) {
+    for (int j = tid; j < 16 * NBT; j += 256) {
         const int64_t jj = min((int64_t)j, g.nt - 1);
-        if constexpr (CV) cthin[j] = g.ctr ? g.c.rofs[jj] : g.c.cofs[jj];
-        else cthin[j] = g.ctr ? jj * g.c.ld : jj;
+        if constexpr (CV) cthin[j] = TC ? g.c.rofs[jj] : g.c.cofs[jj];
+        else cthin[j] = TC ? jj * g.c.ld : jj;
     }
     __syncthreads();
 
@@ -683,19 +694,23 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
             p[1] = 0;
         }
     };
-    // Row of C element r of row block b of unit u in this lane (MFMA
-    // result: lane l, element r holds local row q + 4r, column i16).
-    auto c_row = [&](int64_t u, int b, int r) -> int64_t {
-        return KP ? u * 32 + 16 * b + q + 4 * r : u * 32 + 2 * (q + 4 * r) + b;
+    // MFMA result: lane l, element r holds result row q + 4r, column i16.
+    // The T row of element r of row block b (TC: the same for every r) and
+    // the S row of element r of 16-row block jb of S.
+    auto t_row = [&](int64_t u, int b, int r) -> int64_t {
+        const int loc = TC ? i16 : q + 4 * r;
+        return KP ? u * 32 + 16 * b + loc : u * 32 + 2 * loc + b;
     };
-    auto c_offsets = [&](int64_t u, int64_t (&o)[2][4]) __attribute__((always_inline)) {
+    auto s_row = [&](int jb, int r) -> int { return 16 * jb + (TC ? q + 4 * r : i16); };
+    constexpr int NO = TC ? 1 : 4;  // distinct T rows per row block and lane
+    auto c_offsets = [&](int64_t u, int64_t (&o)[2][NO]) __attribute__((always_inline)) {
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int64_t row = min(c_row(u, b, r), g.rows - 1);
-                if constexpr (CV) o[b][r] = (g.ctr ? g.c.cofs : g.c.rofs)[row];
-                else o[b][r] = g.ctr ? row : row * g.c.ld;
+            for (int r = 0; r < NO; r++) {
+                const int64_t row = min(t_row(u, b, r), g.rows - 1);
+                if constexpr (CV) o[b][r] = (TC ? g.c.cofs : g.c.rofs)[row];
+                else o[b][r] = TC ? row : row * g.c.ld;
             }
     };
     auto load = [&](d2 (&a)[2 * kThinSC], const int64_t (&p)[2], int sc) __attribute__((always_inline)) {
@@ -717,32 +732,37 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
     for (int b = 0; b < 2; b++)
 #pragma unroll
         for (int j = 0; j < NBB; j++) acc[b][j] = (d4){0.0, 0.0, 0.0, 0.0};
-    auto compute = [&](const d2 (&a)[2 * kThinSC], int sc) __attribute__((always_inline)) {
+    auto mma = [&](double tv, double sv, d4 c) -> d4 {
+        if constexpr (TC) return __builtin_amdgcn_mfma_f64_16x16x4f64(sv, tv, c, 0, 0, 0);
+        else return __builtin_amdgcn_mfma_f64_16x16x4f64(tv, sv, c, 0, 0, 0);
+    };
+    auto compute = [&](const d2 (&a)[2 * kThinSC], int sc, int grp) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < kThinSC; c++) {
             const int64_t kc = kThinSC * sc + c;
             if (8 * kc >= K) break;
 #pragma unroll
             for (int j = 0; j < NBB; j++) {
-                const double b0 = bf[((kc * 2) * NBB + j) * 64 + lane];
-                const double b1 = bf[((kc * 2 + 1) * NBB + j) * 64 + lane];
+                const int jb = grp * NBB + j;
+                const double b0 = bf[((kc * 2) * NBT + jb) * 64 + lane];
+                const double b1 = bf[((kc * 2 + 1) * NBT + jb) * 64 + lane];
                 if constexpr (KP) {
-                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].x, b0, acc[0][j], 0, 0, 0);
-                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].x, b0, acc[1][j], 0, 0, 0);
-                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].y, b1, acc[0][j], 0, 0, 0);
-                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].y, b1, acc[1][j], 0, 0, 0);
+                    acc[0][j] = mma(a[2 * c].x, b0, acc[0][j]);
+                    acc[1][j] = mma(a[2 * c + 1].x, b0, acc[1][j]);
+                    acc[0][j] = mma(a[2 * c].y, b1, acc[0][j]);
+                    acc[1][j] = mma(a[2 * c + 1].y, b1, acc[1][j]);
                 } else {
-                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].x, b0, acc[0][j], 0, 0, 0);
-                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c].y, b0, acc[1][j], 0, 0, 0);
-                    acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].x, b1, acc[0][j], 0, 0, 0);
-                    acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * c + 1].y, b1, acc[1][j], 0, 0, 0);
+                    acc[0][j] = mma(a[2 * c].x, b0, acc[0][j]);
+                    acc[1][j] = mma(a[2 * c].y, b0, acc[1][j]);
+                    acc[0][j] = mma(a[2 * c + 1].x, b1, acc[0][j]);
+                    acc[1][j] = mma(a[2 * c + 1].y, b1, acc[1][j]);
                 }
             }
         }
     };
     // Accumulating into C reads a row block's old values with one batch of
     // loads (clamped addresses, all in range) and one wait.
-    auto store = [&](int64_t u, const int64_t (&o)[2][4]) __attribute__((always_inline)) {
+    auto store = [&](int64_t u, const int64_t (&o)[2][NO], int grp) __attribute__((always_inline)) {
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             double old[NBB][4];
@@ -750,17 +770,17 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
 #pragma unroll
                 for (int j = 0; j < NBB; j++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) old[j][r] = g.C[o[b][r] + cthin[16 * j + i16]];
+                    for (int r = 0; r < 4; r++)
+                        old[j][r] = g.C[o[b][TC ? 0 : r] + cthin[s_row(grp * NBB + j, r)]];
             }
 #pragma unroll
             for (int j = 0; j < NBB; j++) {
-                const int64_t col = 16 * j + i16;
-                const int64_t co = cthin[16 * j + i16];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    if (c_row(u, b, r) >= g.rows || col >= g.nt) continue;
+                    const int sr = s_row(grp * NBB + j, r);
+                    if (t_row(u, b, r) >= g.rows || sr >= g.nt) continue;
                     const double v = g.alpha * acc[b][j][r];
-                    g.C[o[b][r] + co] = g.overwrite ? v : old[j][r] + v;
+                    g.C[o[b][TC ? 0 : r] + cthin[sr]] = g.overwrite ? v : old[j][r] + v;
                 }
                 acc[b][j] = (d4){0.0, 0.0, 0.0, 0.0};
             }
@@ -772,7 +792,7 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
     // of the unit after it, loaded by alternate items and used one item
     // later.  Unit indices past the end are clamped, so every item issues
     // the same loads.
-    int64_t p[2], qa[2], qb[2], co[2][4];
+    int64_t p[2], qa[2], qb[2], co[2][NO];
     rows_of(cu, p);
     rows_of(min(cu + NW, U - 1), qb);
     d2 ra[2 * kThinSC], rb[2 * kThinSC];
@@ -793,19 +813,24 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
         rows_of(min(lu + NW, U - 1), qload);
         load(a, p, ls);
     };
+    // One item: every group of S rows (several only with one super chunk).
+    auto work = [&](const d2 (&a)[2 * kThinSC]) __attribute__((always_inline)) {
+        for (int grp = 0; grp < g.ng; grp++) {
+            compute(a, cs, grp);
+            if (cs == nsc - 1) store(cu, co, grp);
+        }
+    };
     while (true) {
         int64_t lu;
         int ls;
         next(rb, qa, qb, lu, ls);
-        compute(ra, cs);
-        if (cs == nsc - 1) store(cu, co);
+        work(ra);
         if (lu >= U) break;
         cu = lu;
         cs = ls;
 
         next(ra, qb, qa, lu, ls);
-        compute(rb, cs);
-        if (cs == nsc - 1) store(cu, co);
+        work(rb);
         if (lu >= U) break;
         cu = lu;
         cs = ls;
@@ -1095,18 +1120,26 @@ int gemv_f64_launch(const GemvArgs &g, hipStream_t s)
     return DTCF_OK;
 }
 
+// 16-row blocks of S a thin product keeps in LDS (0: not a thin shape).
+static int64_t thin_blocks(int64_t w, int64_t k)
+{
+    if (w <= kThinMaxWidth) return (w + 15) / 16;
+    if (w <= kThinMaxWide && k <= kThinWideK) return 3 * ((w + 47) / 48);
+    return 0;
+}
+
 bool thin_shape(int64_t m, int64_t n, int64_t k, int64_t min_rows)
 {
     const int64_t w = std::min(m, n), rows = std::max(m, n);
-    if (w > kThinMaxWidth || rows < min_rows || k <= 0 || k % 8 != 0) return false;
-    const int64_t nbb = (w + 15) / 16;
-    return (k * (16 * nbb + 1) + 16 * nbb) * (int64_t)sizeof(double) <= kThinLdsBytes;
+    const int64_t nbt = thin_blocks(w, k);
+    if (nbt == 0 || rows < min_rows || k <= 0 || k % 8 != 0) return false;
+    return (k * 16 * nbt + k + 16 * nbt) * (int64_t)sizeof(double) <= kThinLdsBytes;
 }
 
-template <bool KP, int NBB, bool TD, bool CV>
+template <bool KP, int NBB, bool TD, bool CV, bool TC>
 int thin_launch_t(const ThinArgs &t, hipStream_t s)
 {
-    auto kern = thin_kernel<KP, NBB, TD, CV>;
+    auto kern = thin_kernel<KP, NBB, TD, CV, TC>;
     static std::once_flag once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(once, [&] {
@@ -1119,31 +1152,38 @@ int thin_launch_t(const ThinArgs &t, hipStream_t s)
     if (st != DTCF_OK) return st;
     // Two resident workgroups per CU (LDS up to 80 KiB each), each wave
     // looping over 32-row units.
-    const int64_t units = (t.rows + 31) / 32;
+    const int64_t units = (t.rows + 31) / 32, nbt = (int64_t)t.ng * NBB;
     const unsigned grid = (unsigned)std::min<int64_t>((units + 3) / 4, (int64_t)cus * 2);
-    const size_t lds = ((size_t)t.k * (16 * NBB + 1) + 16 * NBB) * sizeof(double);
+    const size_t lds = (size_t)(t.k * 16 * nbt + t.k + 16 * nbt) * sizeof(double);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, t);
     DTCF_HIP(hipGetLastError());
     return DTCF_OK;
+}
+
+template <bool KP, bool TD, bool CV, bool TC>
+int thin_dispatch_nbb(const ThinArgs &t, int nbb, hipStream_t s)
+{
+    if (nbb == 1) return thin_launch_t<KP, 1, TD, CV, TC>(t, s);
+    if (nbb == 2) return thin_launch_t<KP, 2, TD, CV, TC>(t, s);
+    return thin_launch_t<KP, 3, TD, CV, TC>(t, s);
 }
 
 template <bool KP, bool TD>
 int thin_dispatch(const ThinArgs &t, int nbb, hipStream_t s)
 {
     if (t.cview) {
-        if (nbb == 1) return thin_launch_t<KP, 1, TD, true>(t, s);
-        if (nbb == 2) return thin_launch_t<KP, 2, TD, true>(t, s);
-        return thin_launch_t<KP, 3, TD, true>(t, s);
+        if (t.ctr) return thin_dispatch_nbb<KP, TD, true, true>(t, nbb, s);
+        return thin_dispatch_nbb<KP, TD, true, false>(t, nbb, s);
     }
-    if (nbb == 1) return thin_launch_t<KP, 1, TD, false>(t, s);
-    if (nbb == 2) return thin_launch_t<KP, 2, TD, false>(t, s);
-    return thin_launch_t<KP, 3, TD, false>(t, s);
+    if (t.ctr) return thin_dispatch_nbb<KP, TD, false, true>(t, nbb, s);
+    return thin_dispatch_nbb<KP, TD, false, false>(t, nbb, s);
 }
 
 int thin_f64_launch(const GemmArgs &g, hipStream_t s)
 {
-    // The long operand: A when C is thin in n, else B (C written transposed).
-    const bool tall_a = g.n <= kThinMaxWidth && g.m >= g.n;
+    // The long operand: A when n is the thin side, else B (C written
+    // transposed).
+    const bool tall_a = g.m >= g.n;
     ThinArgs t{};
     t.rows = tall_a ? g.m : g.n;
     t.nt = tall_a ? g.n : g.m;
@@ -1154,6 +1194,9 @@ int thin_f64_launch(const GemmArgs &g, hipStream_t s)
     const bool kp = tmode == kDK || tmode == kVK;
     if (!to.vec2 || (tmode >= kVK && to.blk) || (!kp && t.rows % 2 != 0)) return kThinDecline;
     if (g.cview && g.c.blk) return kThinDecline;
+    const int64_t nbt = thin_blocks(t.nt, t.k);
+    const int nbb = (int)std::min<int64_t>(nbt, 3);
+    t.ng = (int32_t)(nbt / nbb);
     t.alpha = g.alpha;
     t.T = tall_a ? g.A : g.B;
     t.t = to;
@@ -1166,7 +1209,6 @@ int thin_f64_launch(const GemmArgs &g, hipStream_t s)
     t.cview = g.cview;
     t.ctr = tall_a ? 0 : 1;
     t.overwrite = g.overwrite;
-    const int nbb = (int)((t.nt + 15) / 16);
     if (kp) {
         if (t.tdense) return thin_dispatch<true, true>(t, nbb, s);
         return thin_dispatch<true, false>(t, nbb, s);

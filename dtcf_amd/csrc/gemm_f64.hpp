@@ -18,9 +18,9 @@ enum { kDK = 0, kDMN = 1, kVK = 2, kVMN = 3 };
 // 192x64 and 64x192 serve thin results (one side at most 64) with long K:
 // a 64-wide tile of the thin side, three times as deep on the other, so
 // the thin operand is re-read by a third as many tiles as with 64 x 64.
-// kCfgThin is the streaming kernel for thin results (one side at most 48)
-// with short K (thin_f64_launch); shapes it does not take run 64 x 64
-// tiles.  The grouped launch takes the first four.
+// kCfgThin is the streaming kernel for thin results (one side at most 48,
+// or 192 with K <= 48) with short K (thin_f64_launch); shapes it does not
+// take run 64 x 64 tiles.  The grouped launch takes the first four.
 enum {
     kCfg128x128 = 0,
     kCfg192x96 = 1,
@@ -118,13 +118,16 @@ struct GemvArgs {
 };
 int gemv_f64_launch(const GemvArgs &g, hipStream_t s);
 
-// Thin products: one side of C at most kThinMaxWidth wide, the other long
-// (rows), K a multiple of 8 and short enough that the thin operand's K x
-// width fragments and the long operand's K column offsets fit
-// kThinLdsBytes of LDS.  thin_shape is the shape test; the launch also
-// needs the long operand's pairs adjacent and 16-byte aligned (vec2) and no
-// block table, and returns kThinDecline otherwise.
+// Thin products: one side of C at most kThinMaxWidth wide (or at most
+// kThinMaxWide with K <= kThinWideK), the other long (rows), K a multiple
+// of 8 and short enough that the thin operand's K x width fragments and the
+// offsets fit kThinLdsBytes of LDS.  thin_shape is the shape test; the
+// launch also needs the long operand's pairs adjacent and 16-byte aligned
+// (vec2), no block table on it or on a C view, and returns kThinDecline
+// otherwise.
 constexpr int64_t kThinMaxWidth = 48;
+constexpr int64_t kThinMaxWide = 192;
+constexpr int64_t kThinWideK = 48;
 constexpr int64_t kThinLdsBytes = 80 * 1024;
 constexpr int kThinDecline = 1;
 bool thin_shape(int64_t m, int64_t n, int64_t k, int64_t min_rows);

@@ -379,12 +379,14 @@ def test_gemm_view_random_shapes(dev):
 # C written transposed); long operand with k pairs adjacent (A untransposed,
 # B transposed) or row pairs adjacent, odd long sides (row pairs: the tile
 # path instead), widths 1-48, K from 8 to 200 (the largest at width 48 that
-# fits LDS), and shapes the thin kernel declines (width 49, K not a multiple
-# of 8, K too long).
+# fits LDS), wide thin sides (49-192, K <= 48: several groups of 48), and
+# shapes the thin kernel declines (width 49 with K 64, width 193, K not a
+# multiple of 8, K too long).
 THIN_SHAPES = [(20000, 48, 192, 0, 0), (20000, 48, 192, 1, 1), (3001, 17, 8, 0, 1), (1000, 16, 48, 1, 0),
                (1001, 1, 16, 1, 0), (48, 5000, 200, 0, 0), (40, 3001, 64, 0, 1), (17000, 32, 96, 1, 1),
                (64, 40, 8, 0, 0), (5, 3, 24, 0, 1), (2, 7000, 48, 1, 1), (4000, 49, 64, 0, 0),
-               (3000, 40, 12, 0, 0), (3000, 40, 2048, 0, 1)]
+               (3000, 40, 12, 0, 0), (3000, 40, 2048, 0, 1), (192, 5000, 48, 0, 0), (3000, 100, 16, 0, 1),
+               (150, 4001, 40, 1, 1), (2999, 192, 48, 1, 0), (4000, 193, 48, 0, 0), (64, 3000, 8, 1, 0)]
 
 
 @pytest.mark.parametrize("cfg", [6, -1])
@@ -415,10 +417,11 @@ def test_gemm_thin_views(dev):
     adjacent pairs), C dense with a pitch or a scrambled-row view, thin in n
     or in m, against numpy."""
     rng = np.random.default_rng(4242)
-    shapes = [(3000, 48, 192), (2998, 33, 48), (40, 2500, 200), (1999, 1, 8), (16, 4000, 64), (4001, 20, 8)]
+    shapes = [(3000, 48, 192), (2998, 33, 48), (40, 2500, 200), (1999, 1, 8), (16, 4000, 64), (4001, 20, 8),
+              (192, 3000, 48), (2500, 100, 24)]
     ops.set_gemm_config(6)
     try:
-        for case in range(24):
+        for case in range(32):
             m, n, k = shapes[case % len(shapes)]
             Ah, Ad, av = _random_layout(rng, m, k, dev)
             Bh, Bd, bv = _random_layout(rng, n, k, dev)
