@@ -397,7 +397,7 @@ def test_statement_plans(dev, lists):
                 assert pl["k"] == ek and sorted((pl["m"], pl["n"])) == sorted((ea, eb)), (s.lhs.name, pl)
                 mixed_rule = prec != 0 and pl["m"] * pl["n"] >= 2 * (pl["m"] + pl["n"])
                 assert ("mixed" in pl["flags"]) == mixed_rule, (s.lhs.name, pl)
-                assert pl["tile"] in (("mixed",) if mixed_rule else ("128x128", "64x64", "thin")), pl
+                assert pl["tile"] in (("mixed",) if mixed_rule else ("128x128", "64x64", "192x64", "64x192", "thin")), pl
             if prec:
                 assert any("mixed" in prog.plan(i)["flags"] for i in range(len(tl.statements)))
             prog.destroy()
