@@ -254,6 +254,15 @@ def leg_ccsd(log, o=96, v=384, iters=3):
     return out
 
 
+def leg_thin(log):
+    """The thin FP64 products of the UHF iteration (one side 48 or 192 wide,
+    short K) on the streaming thin kernel and on 64 x 64 tiles."""
+    from dtcf_amd import thinbench
+
+    log("[bench] thin FP64 products (streaming kernel vs 64x64 tiles)")
+    return thinbench.run(reps=5)
+
+
 def leg_dropin(log):
     """The drop-in entry points at the reference's call shapes, from C++
     through the exported symbols (dtcf_amd/dropin_bench, built with the
@@ -586,6 +595,7 @@ def main():
                         ("c1", lambda: leg_c1(dev, log)),
                         ("dropin", lambda: leg_dropin(log)),
                         ("ccsd", lambda: leg_ccsd(log)),
+                        ("thin", lambda: leg_thin(log)),
                         ("precision", lambda: leg_precision(log)),
                         ("blocksparse", lambda: leg_blocksparse(dev, log)),
                         ("serving", lambda: leg_serving(log))):

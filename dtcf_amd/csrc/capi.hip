@@ -395,6 +395,9 @@ static int gemm_view_impl(int64_t m, int64_t n, int64_t k, double alpha, const d
     else { g.bmode = kVK; g.b.vec2 = 0; }
     if (cv) {
         fill_op(g.c, cv);
+        // column pairs of C adjacent and 16-byte aligned (the thin kernel's
+        // paired stores when C runs along the long operand)
+        g.c.vec2 = cv->colpairs && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
         g.cview = 1;
         return gemm_f64_launch(g, pick_config(m, n, k), as_stream(stream));
     }

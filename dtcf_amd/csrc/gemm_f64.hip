@@ -631,6 +631,7 @@ struct ThinArgs {
     double *C;
     OpArgs c;
     int32_t cview, ctr;  // ctr: element (r, j) of the product is C(j, r); views have no block table
+    int32_t cpair;       // ctr: C(j, r), C(j, r + 1) adjacent and 16-byte aligned for even r
     int32_t overwrite;
 };
 
@@ -703,12 +704,15 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
     };
     auto s_row = [&](int jb, int r) -> int { return 16 * jb + (TC ? q + 4 * r : i16); };
     constexpr int NO = TC ? 1 : 4;  // distinct T rows per row block and lane
+    // Rows past the end are clamped into range; with row pairs the even row
+    // clamps to rows - 2 (rows is even), so a clamped pair stays a whole,
+    // aligned pair.
     auto c_offsets = [&](int64_t u, int64_t (&o)[2][NO]) __attribute__((always_inline)) {
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int r = 0; r < NO; r++) {
-                const int64_t row = min(t_row(u, b, r), g.rows - 1);
+                const int64_t row = KP ? min(t_row(u, b, r), g.rows - 1) : min(t_row(u, 0, r), g.rows - 2) + b;
                 if constexpr (CV) o[b][r] = (TC ? g.c.cofs : g.c.rofs)[row];
                 else o[b][r] = TC ? row : row * g.c.ld;
             }
@@ -761,8 +765,40 @@ __global__ __launch_bounds__(256, 2) void thin_kernel(ThinArgs g)
         }
     };
     // Accumulating into C reads a row block's old values with one batch of
-    // loads (clamped addresses, all in range) and one wait.
+    // loads (clamped addresses, all in range) and one wait.  Row pairs in
+    // the transposed form: a lane's two row blocks hold T rows 2i and 2i + 1,
+    // adjacent in C, so with cpair they go out as one 16-byte store (two
+    // 8-byte stores a row apart left half-written lines for L2 to write
+    // back, 1.5x the bytes).
     auto store = [&](int64_t u, const int64_t (&o)[2][NO], int grp) __attribute__((always_inline)) {
+        if constexpr (!KP && TC) {
+            if (g.cpair) {
+                const bool rin = t_row(u, 0, 0) < g.rows;  // rows are even: the pair is whole
+                d2 old2[NBB][4];
+                if (!g.overwrite) {
+#pragma unroll
+                    for (int j = 0; j < NBB; j++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            old2[j][r] = *reinterpret_cast<const d2 *>(
+                                g.C + o[0][0] + cthin[s_row(grp * NBB + j, r)]);
+                }
+#pragma unroll
+                for (int j = 0; j < NBB; j++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int sr = s_row(grp * NBB + j, r);
+                        if (!rin || sr >= g.nt) continue;
+                        d2 v = {g.alpha * acc[0][j][r], g.alpha * acc[1][j][r]};
+                        if (!g.overwrite) v = old2[j][r] + v;
+                        *reinterpret_cast<d2 *>(g.C + o[0][0] + cthin[sr]) = v;
+                    }
+                    acc[0][j] = (d4){0.0, 0.0, 0.0, 0.0};
+                    acc[1][j] = (d4){0.0, 0.0, 0.0, 0.0};
+                }
+                return;
+            }
+        }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             double old[NBB][4];
@@ -1208,6 +1244,9 @@ int thin_f64_launch(const GemmArgs &g, hipStream_t s)
     t.c = g.c;
     t.cview = g.cview;
     t.ctr = tall_a ? 0 : 1;
+    t.cpair = tall_a ? 0
+                     : (g.cview ? g.c.vec2
+                                : ((reinterpret_cast<uintptr_t>(g.C) & 15) == 0 && g.c.ld % 2 == 0));
     t.overwrite = g.overwrite;
     if (kp) {
         if (t.tdense) return thin_dispatch<true, true>(t, nbb, s);
