@@ -349,8 +349,9 @@ static int gemm_view_impl(int64_t m, int64_t n, int64_t k, double alpha, const d
         const bool col = n == 1, row = !col && m == 1;
         const dtcf_view_s *mv = col ? av : bv, *xv = col ? bv : av;
         const double *M = col ? A : B, *X = col ? B : A;
-        if ((col || row) && mv->colpairs && !mv->blk && k % 2 == 0 && k <= kGemvMaxK &&
-            (reinterpret_cast<uintptr_t>(M) & 15) == 0) {
+        const bool aligned = (reinterpret_cast<uintptr_t>(M) & 15) == 0;
+        const bool colwise = mv->colpairs && k % 2 == 0 && k <= kGemvMaxK;
+        if ((col || row) && (colwise || mv->rowpairs) && !mv->blk && aligned) {
             GemvArgs q{};
             q.rows = col ? m : n;
             q.k = k;
@@ -369,7 +370,7 @@ static int gemm_view_impl(int64_t m, int64_t n, int64_t k, double alpha, const d
             } else {
                 q.ldy = ldc;
             }
-            return gemv_f64_launch(q, as_stream(stream));
+            return colwise ? gemv_f64_launch(q, as_stream(stream)) : gemv_rows_f64_launch(q, as_stream(stream));
         }
     }
     GemmArgs g{};

@@ -581,6 +581,66 @@ __global__ __launch_bounds__(256) void gemv_kernel(GemvArgs g)
     }
 }
 
+// Matrix-vector product with M's row pairs adjacent (GemvArgs, mrofs of
+// even rows point at pairs): each lane owns a pair of rows and streams one
+// 16-byte load per column, the column offset and x(c) being the same for
+// the whole wave (scalar loads); a workgroup's 16 waves take 64 row pairs
+// each over their own sixteenth of the columns, in column order, and the
+// sixteen partial sums are added in wave order through LDS: the order of
+// the additions does not depend on timing.
+constexpr int kGemvRowsWaves = 16;
+__global__ __launch_bounds__(64 * kGemvRowsWaves) void gemv_rows_kernel(GemvArgs g)
+{
+    __shared__ d2 red[kGemvRowsWaves][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t npairs = g.rows / 2;
+    const int64_t pr = (int64_t)blockIdx.x * 64 + lane;
+    const double *row = g.M + g.mrofs[2 * min(pr, npairs - 1)];
+    const int64_t kc = (g.k + kGemvRowsWaves - 1) / kGemvRowsWaves;
+    const int64_t c0 = min((int64_t)w * kc, g.k), c1 = min(c0 + kc, g.k);
+    d2 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[u] = (d2){0.0, 0.0};
+    int64_t c = c0;
+    for (; c + 4 <= c1; c += 4) {
+        d2 v[4];
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            v[u] = *reinterpret_cast<const d2 *>(row + g.mcofs[c + u]);
+            x[u] = g.X[op_addr<kVK>(g.xv, 0, c + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            acc[u].x = fma(v[u].x, x[u], acc[u].x);
+            acc[u].y = fma(v[u].y, x[u], acc[u].y);
+        }
+    }
+    for (int u = 0; c < c1; c++, u++) {
+        const d2 v = *reinterpret_cast<const d2 *>(row + g.mcofs[c]);
+        const double x = g.X[op_addr<kVK>(g.xv, 0, c)];
+        acc[u].x = fma(v.x, x, acc[u].x);
+        acc[u].y = fma(v.y, x, acc[u].y);
+    }
+    red[w][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (w != 0 || pr >= npairs) return;
+    d2 sum = red[0][lane];
+#pragma unroll
+    for (int q = 1; q < kGemvRowsWaves; q++) sum += red[q][lane];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int64_t r = 2 * pr + h;
+        const double v = g.alpha * (h ? sum.y : sum.x);
+        int64_t addr;
+        if (g.yview) addr = g.ycol ? op_addr<kVK>(g.yv, r, 0) : op_addr<kVK>(g.yv, 0, r);
+        else addr = g.ycol ? r * g.ldy : r;
+        if (g.overwrite) g.Y[addr] = v;
+        else g.Y[addr] += v;
+    }
+}
+
 // Thin products (ThinArgs): C (rows x nt) = alpha * T S^T with T the long
 // operand (rows x k, k short) and S the thin one (nt x k; nt <= 48, or up
 // to 192 when k <= 48).  Every element of T is used once, so the product
@@ -1254,6 +1314,17 @@ int thin_f64_launch(const GemmArgs &g, hipStream_t s)
     }
     if (t.tdense) return thin_dispatch<false, true>(t, nbb, s);
     return thin_dispatch<false, false>(t, nbb, s);
+}
+
+int gemv_rows_f64_launch(const GemvArgs &g, hipStream_t s)
+{
+    DTCF_REQUIRE(g.k > 0 && g.rows % 2 == 0, "gemv rows: k must be positive and the row count even");
+    if (g.rows <= 0) return DTCF_OK;
+    const int64_t blocks = (g.rows / 2 + 63) / 64;
+    DTCF_REQUIRE(blocks < (1ll << 31), "gemv rows: too many rows");
+    hipLaunchKernelGGL(gemv_rows_kernel, dim3((unsigned)blocks), dim3(64 * kGemvRowsWaves), 0, s, g);
+    DTCF_HIP(hipGetLastError());
+    return DTCF_OK;
 }
 
 int gemm_workspace(hipStream_t s, size_t need, double **ws) { return split_workspace(s, need, ws); }

@@ -117,6 +117,9 @@ struct GemvArgs {
     int64_t ldy;
 };
 int gemv_f64_launch(const GemvArgs &g, hipStream_t s);
+// The same product when M has its row pairs adjacent instead (rofs[r + 1] =
+// rofs[r] + 1 for even r, offsets even, rows even, M 16-byte aligned): any k.
+int gemv_rows_f64_launch(const GemvArgs &g, hipStream_t s);
 
 // Thin products: one side of C at most kThinMaxWidth wide (or at most
 // kThinMaxWide with K <= kThinWideK), the other long (rows), K a multiple

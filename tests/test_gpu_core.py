@@ -259,16 +259,18 @@ def test_gemm_group_ex_layouts(dev, transa, transb, cfg):
 @pytest.mark.parametrize("orient", ["col", "row"])
 @pytest.mark.parametrize("rows,k,layout", [(5000, 9216, "pairs"), (37, 130, "pairs"), (300, 4, "pairs"),
                                            (700, 20482, "pairs"), (900, 515, "pairs"),
-                                           (800, 256, "rows")])
+                                           (800, 256, "rows"), (5000, 9216, "rows"), (130, 7, "rows"),
+                                           (2, 40000, "rows"), (37, 130, "rows")])
 @pytest.mark.parametrize("cview", [False, True])
 def test_gemm_view_matrix_vector(dev, orient, rows, k, layout, cview):
     """Matrix-vector products through the view GEMM (n = 1: C[:, 0] = alpha
     A x; m = 1: C[0, :] = alpha x B^T).  The long operand's rows are
     scrambled (a row offset table), its column pairs adjacent ('pairs': the
     streaming path when k is even and at most 20480; k = 20482 and 515 take
-    the GEMM) or its rows adjacent ('rows': the GEMM); x is read through a
+    the GEMM) or its rows adjacent ('rows'); x is read through a
     strided view; C is dense or a scrambled view; accumulate into C, against
-    numpy."""
+    numpy.  Row pairs adjacent with an even row count take the row-pair
+    streaming kernel (any k); an odd row count takes the GEMM."""
     rng = np.random.default_rng(rows + k)
     Mh = rng.standard_normal((rows, k))
     xh = rng.standard_normal(k)
