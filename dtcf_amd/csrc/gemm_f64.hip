@@ -585,11 +585,22 @@ __global__ __launch_bounds__(256) void gemv_kernel(GemvArgs g)
 // even rows point at pairs): each lane owns a pair of rows and streams one
 // 16-byte load per column, the column offset and x(c) being the same for
 // the whole wave (scalar loads); a workgroup's 16 waves take 64 row pairs
-// each over their own sixteenth of the columns, in column order, and the
-// sixteen partial sums are added in wave order through LDS: the order of
-// the additions does not depend on timing.
+// each over their own sixteenth of the workgroup's columns, in column
+// order, and the sixteen partial sums are added in wave order through LDS.
+// With few rows the columns are also split over gridDim.y workgroups, whose
+// partial sums (ws[slice][row]) gemv_rows_reduce adds in slice order: the
+// order of the additions never depends on timing.
 constexpr int kGemvRowsWaves = 16;
-__global__ __launch_bounds__(64 * kGemvRowsWaves) void gemv_rows_kernel(GemvArgs g)
+__device__ inline void gemv_store(const GemvArgs &g, int64_t r, double v)
+{
+    int64_t addr;
+    if (g.yview) addr = g.ycol ? op_addr<kVK>(g.yv, r, 0) : op_addr<kVK>(g.yv, 0, r);
+    else addr = g.ycol ? r * g.ldy : r;
+    if (g.overwrite) g.Y[addr] = v;
+    else g.Y[addr] += v;
+}
+
+__global__ __launch_bounds__(64 * kGemvRowsWaves) void gemv_rows_kernel(GemvArgs g, double *ws)
 {
     __shared__ d2 red[kGemvRowsWaves][64];
     const int lane = threadIdx.x & 63;
@@ -597,8 +608,10 @@ __global__ __launch_bounds__(64 * kGemvRowsWaves) void gemv_rows_kernel(GemvArgs
     const int64_t npairs = g.rows / 2;
     const int64_t pr = (int64_t)blockIdx.x * 64 + lane;
     const double *row = g.M + g.mrofs[2 * min(pr, npairs - 1)];
-    const int64_t kc = (g.k + kGemvRowsWaves - 1) / kGemvRowsWaves;
-    const int64_t c0 = min((int64_t)w * kc, g.k), c1 = min(c0 + kc, g.k);
+    const int64_t ks = (g.k + gridDim.y - 1) / gridDim.y;  // columns of this workgroup's slice
+    const int64_t b0 = min((int64_t)blockIdx.y * ks, g.k), b1 = min(b0 + ks, g.k);
+    const int64_t kc = (b1 - b0 + kGemvRowsWaves - 1) / kGemvRowsWaves;
+    const int64_t c0 = min(b0 + (int64_t)w * kc, b1), c1 = min(c0 + kc, b1);
     d2 acc[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) acc[u] = (d2){0.0, 0.0};
@@ -629,16 +642,19 @@ __global__ __launch_bounds__(64 * kGemvRowsWaves) void gemv_rows_kernel(GemvArgs
     d2 sum = red[0][lane];
 #pragma unroll
     for (int q = 1; q < kGemvRowsWaves; q++) sum += red[q][lane];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int64_t r = 2 * pr + h;
-        const double v = g.alpha * (h ? sum.y : sum.x);
-        int64_t addr;
-        if (g.yview) addr = g.ycol ? op_addr<kVK>(g.yv, r, 0) : op_addr<kVK>(g.yv, 0, r);
-        else addr = g.ycol ? r * g.ldy : r;
-        if (g.overwrite) g.Y[addr] = v;
-        else g.Y[addr] += v;
+    if (gridDim.y > 1) {
+        *reinterpret_cast<d2 *>(ws + (int64_t)blockIdx.y * g.rows + 2 * pr) = sum;
+        return;
     }
+    gemv_store(g, 2 * pr, g.alpha * sum.x);
+    gemv_store(g, 2 * pr + 1, g.alpha * sum.y);
+}
+
+__global__ __launch_bounds__(256) void gemv_rows_reduce(GemvArgs g, const double *ws, uint32_t nslice)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= g.rows) return;
+    gemv_store(g, r, g.alpha * sum_in_order(ws + r, nslice, g.rows));
 }
 
 // Thin products (ThinArgs): C (rows x nt) = alpha * T S^T with T the long
@@ -1322,8 +1338,26 @@ int gemv_rows_f64_launch(const GemvArgs &g, hipStream_t s)
     if (g.rows <= 0) return DTCF_OK;
     const int64_t blocks = (g.rows / 2 + 63) / 64;
     DTCF_REQUIRE(blocks < (1ll << 31), "gemv rows: too many rows");
-    hipLaunchKernelGGL(gemv_rows_kernel, dim3((unsigned)blocks), dim3(64 * kGemvRowsWaves), 0, s, g);
+    int cus = 0;
+    int st = device_cus(&cus);
+    if (st != DTCF_OK) return st;
+    // Column slices over workgroups until the grid covers two workgroups
+    // per CU, each slice keeping at least 64 columns per wave.
+    int64_t nslice = 1;
+    while (blocks * nslice < 2 * (int64_t)cus && g.k / (2 * nslice) >= 64 * kGemvRowsWaves) nslice *= 2;
+    double *ws = nullptr;
+    if (nslice > 1) {
+        st = split_workspace(s, (size_t)nslice * g.rows * sizeof(double), &ws);
+        if (st != DTCF_OK) return st;
+    }
+    hipLaunchKernelGGL(gemv_rows_kernel, dim3((unsigned)blocks, (unsigned)nslice), dim3(64 * kGemvRowsWaves), 0,
+                       s, g, ws);
     DTCF_HIP(hipGetLastError());
+    if (nslice > 1) {
+        hipLaunchKernelGGL(gemv_rows_reduce, dim3((unsigned)((g.rows + 255) / 256)), dim3(256), 0, s, g,
+                           (const double *)ws, (uint32_t)nslice);
+        DTCF_HIP(hipGetLastError());
+    }
     return DTCF_OK;
 }
 
