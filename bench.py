@@ -227,7 +227,7 @@ def leg_ccsd(log, o=96, v=384, iters=3):
                      sigma=0.5 / v, log=lambda s_: log("[ccsd] " + s_))
     out = {"workload": f"UHF CCSD iterations o={o} v={v} (Oa=Ob={o // 2}, Va=Vb={v // 2}), "
                        f"sigma={0.5 / v:.3g}, {iters} iterations from the first-order amplitudes, "
-                       f"every mode from the FP64 iteration's amplitudes",
+                       f"every mode from the FP64 iteration's amplitudes, one server slot",
            "flops_per_iteration": res["flops_per_iteration"], "modes": {}}
     for m, r in res["modes"].items():
         out["modes"][m] = {"seconds_per_iteration": round(r["seconds_per_iteration"], 4),

@@ -223,7 +223,7 @@ MODES = {
 }
 
 
-def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb=240.0, nslots=8,
+def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb=240.0, nslots=1,
           seed=11, sigma=0.05, log=print):
     """Full UHF CCSD iterations, FP64 against mixed residual programs.
 
@@ -235,7 +235,10 @@ def study(o=64, v=256, iters=3, modes=("fp64", "int8x5", "int8x4/L6"), budget_gb
     the update / energy step), contraction TFLOP/s (dense flops of the two
     programs over the residual time), speedup over FP64, and the largest
     |dE| (total and per component E1, Eaa, Eab, Ebb) and max |dR| / max |R|
-    per residual block over the iterations."""
+    per residual block over the iterations.  One server slot by default: the
+    programs' large kernels fill the GPU by themselves, and at o=96 v=384 one
+    slot ran both forms fastest (FP64 1.242 s, INT8x3/L4 0.320 s per
+    iteration, against 1.287 s and 0.328 s on eight)."""
     srv = serve.Server(int(budget_gb * (1 << 30)), nslots)
     try:
         t = time.perf_counter()
@@ -312,11 +315,12 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=240.0)
     ap.add_argument("--sigma", type=float, default=0.05)
     ap.add_argument("--modes", default="fp64,int8x5,int8x4/L6")
+    ap.add_argument("--nslots", type=int, default=1)
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     check(lib().dtcf_init(0))
     res = study(args.o, args.v, args.iters, tuple(args.modes.split(",")), args.budget_gb,
-                sigma=args.sigma, log=lambda s: print(s, flush=True))
+                nslots=args.nslots, sigma=args.sigma, log=lambda s: print(s, flush=True))
     print(json.dumps(res), flush=True)
     if args.out:
         with open(args.out, "w") as f:
