@@ -28,6 +28,11 @@
 //
 // Workgroup ids are remapped so consecutive C tiles of a row group land on
 // the same XCD (its own L2), then swizzled in groups of 8 tile rows.
+//
+// Shapes the tiles serve badly have streaming kernels of their own, chosen
+// by the launch: matrix-vector products (gemv_kernel, gemv_rows_kernel) and
+// thin products with one side at most 48 (192 with short K) and short K
+// (thin_kernel).  Each is bound by streaming its long operand from HBM.
 #include <algorithm>
 #include <map>
 #include <mutex>
