@@ -152,7 +152,13 @@ __global__ __launch_bounds__(kPlaneThreads, 4) void sum_plane_kernel(SumArgs a, 
         i2[k] = div_d3.div(e);
         i3[k] = e - i2[k] * d3;
     }
-    for (uint64_t plane = blockIdx.x; plane < nplanes; plane += gridDim.x) {
+    // Workgroups go to the XCDs in turn; when the grid allows, consecutive
+    // planes are given to workgroups of one XCD, so terms whose neighbouring
+    // planes share cache lines (the plane's indices outermost in the term)
+    // find those lines in that XCD's L2.
+    uint64_t first = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) first = (uint64_t)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    for (uint64_t plane = first; plane < nplanes; plane += gridDim.x) {
         const uint64_t p0 = plane / (uint64_t)a.d1, p1 = plane - p0 * (uint64_t)a.d1;
         double acc[NS][W];
 #pragma unroll
